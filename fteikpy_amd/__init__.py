@@ -1,0 +1,43 @@
+"""fteikpy_amd: MI355X-native fast-sweeping eikonal solver with fteikpy's interface.
+
+Drop-in names of fteikpy/__init__.py:1-19.  Solves, interpolation, gradients and ray
+tracing run as hand-written HIP kernels (gfx950) through the C ABI of
+include/fteik_amd.h; arrays live in device memory as torch tensors.
+"""
+from ._grid import Grid2D, Grid3D, TraveltimeGrid2D, TraveltimeGrid3D
+from ._helpers import get_num_threads, set_num_threads
+from ._lib import build as _build_lib
+from ._solver import Eikonal2D, Eikonal3D
+
+__version__ = "0.1.0"
+
+
+def build():
+    """Compile the HIP library (libfteik_amd.so, gfx950) in-tree."""
+    _build_lib()
+
+
+def grid_to_meshio(*args):
+    """meshio export (fteikpy/_io.py:7-106) is not part of this build."""
+    raise NotImplementedError("meshio export is not implemented in fteikpy_amd")
+
+
+def ray_to_meshio(*args):
+    """meshio export (fteikpy/_io.py:109-153) is not part of this build."""
+    raise NotImplementedError("meshio export is not implemented in fteikpy_amd")
+
+
+__all__ = [
+    "Eikonal2D",
+    "Eikonal3D",
+    "Grid2D",
+    "Grid3D",
+    "TraveltimeGrid2D",
+    "TraveltimeGrid3D",
+    "get_num_threads",
+    "set_num_threads",
+    "grid_to_meshio",
+    "ray_to_meshio",
+    "build",
+    "__version__",
+]

@@ -1,0 +1,345 @@
+"""Device-resident grid objects with the reference's interface.
+
+Mirrors fteikpy/_base.py:10-241 (BaseGrid, BaseGrid2D/3D, BaseTraveltime) and
+fteikpy/_grid.py:8-427 (Grid2D/3D, TraveltimeGrid2D/3D).  Arrays live in HIP device memory
+as torch float64 tensors; ``.grid`` returns a host numpy copy (as the reference returns
+numpy) and ``.tensor`` the device tensor.  Interpolation, apparent-velocity traveltime
+interpolation, gradients and ray tracing run as HIP kernels through the C ABI.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def device():
+    """The current HIP device; raises when there is none (no CPU fallback)."""
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise RuntimeError("fteikpy_amd needs a HIP device (MI355X); none is available")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_ptr():
+    return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def to_device(a, dtype=None):
+    """float64 (or `dtype`) tensor on the current device; views of device tensors are kept."""
+    torch = _torch()
+    dtype = dtype or torch.float64
+    if isinstance(a, torch.Tensor):
+        if a.device.type == "cuda" and a.dtype == dtype:
+            return a
+        return a.to(device=device(), dtype=dtype)
+    return torch.as_tensor(np.asarray(a, dtype=np.float64), device=device()).to(dtype)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _element_stride(t):
+    """Element stride k such that t[i0,...] sits at ((i0*n1 + i1)*n2 + ...)*k, or None."""
+    k = t.stride(-1)
+    expect = k
+    for n, s in zip(reversed(t.shape), reversed(t.stride())):
+        if s != expect:
+            return None
+        expect *= n
+    return k
+
+
+class BaseGrid:
+    """Grid storage and properties, fteikpy/_base.py:10-50."""
+
+    _ndim = None
+
+    def __init__(self, grid, gridsize, origin, **kwargs):
+        super().__init__(**kwargs)
+        self._dev = to_device(grid)
+        self._host = None
+        self._gridsize = tuple(float(x) for x in gridsize)
+        self._origin = np.asarray(origin, dtype=np.float64)
+
+    def __getitem__(self, islice):
+        return self.grid[islice]
+
+    @property
+    def grid(self):
+        """Grid values as a host numpy array (copied from the device once)."""
+        if self._host is None:
+            self._host = self._dev.detach().cpu().numpy()
+        return self._host
+
+    @property
+    def tensor(self):
+        """Grid values as a device tensor."""
+        return self._dev
+
+    @property
+    def gridsize(self):
+        return self._gridsize
+
+    @property
+    def origin(self):
+        return self._origin
+
+    @property
+    def shape(self):
+        return tuple(self._dev.shape)
+
+    @property
+    def size(self):
+        return int(self._dev.numel())
+
+    @property
+    def ndim(self):
+        return self._dev.dim()
+
+    def _axis(self, a):
+        return self._origin[a] + self._gridsize[a] * np.arange(self.shape[a])
+
+    def _strided(self):
+        """(tensor, element stride) usable by the interpolation kernels."""
+        k = _element_stride(self._dev)
+        if k is None:
+            return self._dev.contiguous(), 1
+        return self._dev, k
+
+    def _interp(self, points, fill_value):
+        torch = _torch()
+        nd = self._ndim
+        q = np.asarray(points, dtype=np.float64)
+        single = q.ndim == 1
+        q2 = q.reshape(-1, nd)
+        qd = torch.as_tensor(np.ascontiguousarray(q2), device=device())
+        out = torch.empty(len(q2), dtype=torch.float64, device=device())
+        v, k = self._strided()
+        o, op = _lib.host_f8(self._origin)
+        d, dp = _lib.host_f8(self._gridsize)
+        L = _lib.lib()
+        if nd == 3:
+            st = L.fteik3d_interp(_ptr(v), *self.shape, k, op, dp, _ptr(qd), len(q2),
+                                  float(fill_value), _ptr(out), stream_ptr())
+        else:
+            st = L.fteik2d_interp(_ptr(v), *self.shape, k, op, dp, _ptr(qd), len(q2),
+                                  float(fill_value), _ptr(out), stream_ptr())
+        _lib.check(st)
+        res = out.cpu().numpy()
+        return float(res[0]) if single else res
+
+
+class BaseGrid2D(BaseGrid):
+    """fteikpy/_base.py:53-133"""
+
+    _ndim = 2
+
+    def __call__(self, points, fill_value=np.nan):
+        """Bilinear interpolation (fteikpy/_base.py:56-79)."""
+        return self._interp(points, fill_value)
+
+    @property
+    def zaxis(self):
+        return self._axis(0)
+
+    @property
+    def xaxis(self):
+        return self._axis(1)
+
+
+class BaseGrid3D(BaseGrid):
+    """fteikpy/_base.py:136-227"""
+
+    _ndim = 3
+
+    def __call__(self, points, fill_value=np.nan):
+        """Trilinear interpolation (fteikpy/_base.py:139-163)."""
+        return self._interp(points, fill_value)
+
+    @property
+    def zaxis(self):
+        return self._axis(0)
+
+    @property
+    def xaxis(self):
+        return self._axis(1)
+
+    @property
+    def yaxis(self):
+        return self._axis(2)
+
+
+class BaseTraveltime:
+    """fteikpy/_base.py:230-241, plus the device sign grid the gradient is built from."""
+
+    def __init__(self, source, gradient, vzero, sign=None, sets_done=None, **kwargs):
+        super().__init__(**kwargs)
+        self._source = np.asarray(source, dtype=np.float64)
+        self._gradient_dev = to_device(gradient) if gradient is not None else None
+        self._sign = sign
+        self._vzero = float(vzero)
+        self._sets_done = sets_done
+
+    @property
+    def source(self):
+        return self._source
+
+    @property
+    def vzero(self):
+        """Slowness of the source cell."""
+        return self._vzero
+
+    @property
+    def sets_done(self):
+        """Sweep sets executed by the solver (the last one changed no node if converged)."""
+        return self._sets_done
+
+    def _rel_source(self):
+        return self._source - self._origin
+
+    def _gradient_tensor(self):
+        if self._gradient_dev is None:
+            if self._sign is None:
+                raise ValueError(
+                    "no gradient grid, use option `return_gradient` to return gradient grids")
+            torch = _torch()
+            nd = self._ndim
+            g = torch.empty(self.shape + (nd,), dtype=torch.float64, device=device())
+            cells = [n - 1 for n in self.shape]
+            src, sp = _lib.host_f8(self._rel_source())
+            tt = self._dev.contiguous()
+            L = _lib.lib()
+            if nd == 3:
+                st = L.fteik3d_gradient(_ptr(tt), _ptr(self._sign), *cells, *self._gridsize, sp,
+                                        self._vzero, _ptr(g), stream_ptr())
+            else:
+                st = L.fteik2d_gradient(_ptr(tt), _ptr(self._sign), *cells, *self._gridsize, sp,
+                                        self._vzero, _ptr(g), stream_ptr())
+            _lib.check(st)
+            self._gradient_dev = g
+        return self._gradient_dev
+
+    def _vinterp(self, points, fill_value):
+        torch = _torch()
+        nd = self._ndim
+        q = np.asarray(points, dtype=np.float64)
+        single = q.ndim == 1
+        q2 = np.ascontiguousarray(q.reshape(-1, nd))
+        qd = torch.as_tensor(q2, device=device())
+        out = torch.empty(len(q2), dtype=torch.float64, device=device())
+        tt = self._dev.contiguous()
+        o, op = _lib.host_f8(self._origin)
+        d, dp = _lib.host_f8(self._gridsize)
+        s, sp = _lib.host_f8(self._source)
+        L = _lib.lib()
+        fn = L.fteik3d_vinterp if nd == 3 else L.fteik2d_vinterp
+        st = fn(_ptr(tt), *self.shape, op, dp, _ptr(qd), len(q2), sp, self._vzero,
+                float(fill_value), _ptr(out), stream_ptr())
+        _lib.check(st)
+        res = out.cpu().numpy()
+        return float(res[0]) if single else res
+
+    def _raytrace(self, points, stepsize, max_step, honor_grid):
+        torch = _torch()
+        nd = self._ndim
+        grad = self._gradient_tensor()
+        if honor_grid or not stepsize:
+            stepsize = np.min(self._gridsize)
+        if not max_step:
+            max_dist = 2.0 * sum((n * d) ** 2 for n, d in zip(self.shape, self._gridsize)) ** 0.5
+            max_step = int(max_dist / stepsize)
+        p = np.asarray(points, dtype=np.float64)
+        single = p.ndim == 1
+        p2 = np.ascontiguousarray(p.reshape(-1, nd))
+        n = len(p2)
+        pd = torch.as_tensor(p2, device=device())
+        rays = torch.empty((n, int(max_step), nd), dtype=torch.float64, device=device())
+        counts = torch.empty(n, dtype=torch.int32, device=device())
+        status = torch.empty(n, dtype=torch.int32, device=device())
+        o, op = _lib.host_f8(self._origin)
+        d, dp = _lib.host_f8(self._gridsize)
+        s, sp = _lib.host_f8(self._source)
+        L = _lib.lib()
+        fn = L.fteik3d_raytrace if nd == 3 else L.fteik2d_raytrace
+        st = fn(_ptr(grad.contiguous()), *self.shape, op, dp, _ptr(pd), n, sp, float(stepsize),
+                int(max_step), int(bool(honor_grid)), _ptr(rays), _ptr(counts), _ptr(status),
+                stream_ptr())
+        _lib.check(st)
+        status = status.cpu().numpy()
+        for code in (_lib.ERR_ENDPOINT_OOB, _lib.ERR_MAX_STEP):
+            if (status == code).any():
+                _lib.check(code)
+        counts = counts.cpu().numpy()
+        # only the used prefix of each ray comes back; returned source first
+        lmax = int(counts.max()) + 1 if n else 0
+        host = rays[:, :lmax].cpu().numpy()
+        out = [host[r, counts[r]::-1].copy() for r in range(n)]
+        return out[0] if single else out
+
+
+class Grid2D(BaseGrid2D):
+    """2D grid, fteikpy/_grid.py:8-23."""
+
+    def __init__(self, grid, gridsize, origin):
+        super().__init__(grid=grid, gridsize=gridsize, origin=origin)
+
+
+class Grid3D(BaseGrid3D):
+    """3D grid, fteikpy/_grid.py:26-41."""
+
+    def __init__(self, grid, gridsize, origin):
+        super().__init__(grid=grid, gridsize=gridsize, origin=origin)
+
+
+class TraveltimeGrid2D(BaseGrid2D, BaseTraveltime):
+    """2D traveltime grid, fteikpy/_grid.py:193-307."""
+
+    def __init__(self, grid, gridsize, origin, source, gradient, vzero, sign=None, sets_done=None):
+        super().__init__(grid=grid, gridsize=gridsize, origin=np.asarray(origin, dtype=np.float64),
+                         source=source, gradient=gradient, vzero=vzero, sign=sign,
+                         sets_done=sets_done)
+
+    def __call__(self, points, fill_value=np.nan):
+        """Bilinear apparent velocity interpolation (fteikpy/_grid.py:225-250)."""
+        return self._vinterp(points, fill_value)
+
+    def raytrace(self, points, stepsize=None, max_step=None, honor_grid=False):
+        """2D a posteriori ray-tracing (fteikpy/_grid.py:252-294)."""
+        return self._raytrace(points, stepsize, max_step, honor_grid)
+
+    @property
+    def gradient(self):
+        """Z and X gradient grids as a list of Grid2D (fteikpy/_grid.py:296-307)."""
+        g = self._gradient_tensor()
+        return [Grid2D(g[:, :, i], self._gridsize, self._origin) for i in range(2)]
+
+
+class TraveltimeGrid3D(BaseGrid3D, BaseTraveltime):
+    """3D traveltime grid, fteikpy/_grid.py:310-427."""
+
+    def __init__(self, grid, gridsize, origin, source, gradient, vzero, sign=None, sets_done=None):
+        super().__init__(grid=grid, gridsize=gridsize, origin=np.asarray(origin, dtype=np.float64),
+                         source=source, gradient=gradient, vzero=vzero, sign=sign,
+                         sets_done=sets_done)
+
+    def __call__(self, points, fill_value=np.nan):
+        """Trilinear apparent velocity interpolation (fteikpy/_grid.py:342-368)."""
+        return self._vinterp(points, fill_value)
+
+    def raytrace(self, points, stepsize=None, max_step=None, honor_grid=False):
+        """3D a posteriori ray-tracing (fteikpy/_grid.py:370-414)."""
+        return self._raytrace(points, stepsize, max_step, honor_grid)
+
+    @property
+    def gradient(self):
+        """Z, X and Y gradient grids as a list of Grid3D (fteikpy/_grid.py:416-427)."""
+        g = self._gradient_tensor()
+        return [Grid3D(g[:, :, :, i], self._gridsize, self._origin) for i in range(3)]

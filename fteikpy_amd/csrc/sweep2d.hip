@@ -1,0 +1,407 @@
+// 2D fast-sweeping eikonal solver for MI355X (gfx950): FTeik 2D operators with the
+// near-source spherical (factored) operator (fteikpy/_fteik/_fteik2d.py:33-187), the
+// off-grid source line initialisation (_fteik2d.py:355-617) and the column-interleaved
+// 4-direction sweep set (_fteik2d.py:190-287).
+//
+// The reference's 2D set is one serial dependency chain: column j's +z pass reads column
+// j-1 after its -z pass.  Exact parity at any nsweep therefore needs the reference's
+// order, and each source is swept by one lane; sources run in parallel.
+#include "fteik_amd.h"
+#include "fteik_device.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace fteik {
+namespace {
+
+struct Grid2 {
+    const double *slow;  // [nz][nx] cells
+    double *tt;          // [NZ][NX] nodes
+    uint8_t *sgn;        // [NZ][NX] or null
+    int nz, nx, NZ, NX;
+    double dz, dx, dzi, dxi, dz2i, dx2i;
+    __device__ __forceinline__ double &T(int i, int j) const { return tt[(long long)i * NX + j]; }
+    __device__ __forceinline__ double S(int i, int j) const { return slow[(long long)i * nx + j]; }
+    __device__ __forceinline__ void setsgn(int i, int j, int sz, int sx) const {
+        if (sgn) sgn[(long long)i * NX + j] = (uint8_t)(sgn_code(sz) | (sgn_code(sx) << 2));
+    }
+};
+
+// _fteik2d.py:11-14
+__device__ __forceinline__ double tana(int i, int j, double dz, double dx, double zsa, double xsa,
+                                       double vzero) {
+    const double a = dz * ((double)i - zsa), b = dx * ((double)j - xsa);
+    return vzero * sqrt(a * a + b * b);
+}
+
+// _fteik2d.py:17-30: analytic time and its derivatives
+__device__ __forceinline__ double tanad(int i, int j, double dz, double dx, double zsa,
+                                        double xsa, double vzero, double *tzc, double *txc) {
+    const double t = tana(i, j, dz, dx, zsa, xsa, vzero);
+    if (t > 0.0) {
+        const double tmp = vzero * vzero / t;
+        *tzc = ((double)i - zsa) * dz * tmp;
+        *txc = ((double)j - xsa) * dx * tmp;
+    } else {
+        *tzc = 0.0;
+        *txc = 0.0;
+    }
+    return t;
+}
+
+// Factored quadratic of the spherical operator, _fteik2d.py:33-67.
+__device__ __forceinline__ double delta(double t1, double tauv, double taue, double tauev,
+                                        double t0c, double tzc, double txc, double dzi,
+                                        double dxi, double dz2i, double dx2i, double vzero,
+                                        double vref, int sgntz, int sgntx) {
+    const double ta = tauev + taue - tauv;
+    const double tb = tauev - taue + tauv;
+    const double apoly = dz2i + dx2i;
+    const double bpoly = 4.0 * ((double)sgntx * txc * dxi + (double)sgntz * tzc * dzi) -
+                         2.0 * (ta * dx2i + tb * dz2i);
+    const double cpoly = ((ta * ta) * dx2i) + ((tb * tb) * dz2i) -
+                         4.0 * ((double)sgntx * txc * dxi * ta + (double)sgntz * tzc * dzi * tb) +
+                         4.0 * (vzero * vzero - vref * vref);
+    const double dpoly = bpoly * bpoly - 4.0 * apoly * cpoly;
+    return dpoly >= 0.0 ? 0.5 * (sqrt(dpoly) - bpoly) / apoly + t0c : t1;
+}
+
+// 2D node update, _fteik2d.py:70-187.  Returns true if the node changed.
+__device__ inline bool node2(const Grid2 &G, const SrcParam &P, double vzero, int i, int j,
+                             int sgnvz, int sgnvx, int sgntz, int sgntx) {
+    const int NZ = G.NZ, NX = G.NX;
+    const double dz = G.dz, dx = G.dx;
+    const int i1 = i - sgnvz, j1 = j - sgnvx;
+    const double tv = G.T(i - sgntz, j);
+    const double te = G.T(i, j - sgntx);
+    const double tev = G.T(i - sgntz, j - sgntx);
+
+    double vref = pymin(G.S(i1, max(j - 1, 0)), G.S(i1, min(j, NX - 2)));
+    const double t1d1 = tv + dz * vref;
+    vref = pymin(G.S(max(i - 1, 0), j1), G.S(min(i, NZ - 2), j1));
+    const double t1d2 = te + dx * vref;
+    const double t1d = pymin(t1d1, t1d2);
+
+    double t2d = kBig;
+    vref = G.S(i1, j1);
+    if (abs(i - P.zsi) > kEpsIn || abs(j - P.xsi) > kEpsIn) {
+        // plane-wave: 4-point operator, else one of two 3-point operators
+        if (tv <= te + dx * vref && te <= tv + dz * vref && te >= tev && tv >= tev) {
+            const double ta = tev + te - tv, tb = tev - te + tv, q = ta - tb;
+            t2d = ((tb * G.dz2i + ta * G.dx2i) +
+                   sqrt(4.0 * (vref * vref) * (G.dz2i + G.dx2i) - G.dz2i * G.dx2i * (q * q))) /
+                  (G.dz2i + G.dx2i);
+        } else if (te - tev <= dz * dz * vref / sqrt(dx * dx + dz * dz) && te - tev > 0.0) {
+            const double r = (te - tev) / dz;
+            t2d = te + dx * sqrt(vref * vref - r * r);
+        } else if (tv - tev <= dx * dx * vref / sqrt(dx * dx + dz * dz) && tv - tev > 0.0) {
+            const double r = (tv - tev) / dx;
+            t2d = tv + dz * sqrt(vref * vref - r * r);
+        }
+    } else {
+        // spherical operator near the source
+        if (tv < te + dx * vref && te < tv + dz * vref && te >= tev && tv >= tev) {
+            double tzc, txc;
+            const double t0c = tanad(i, j, dz, dx, P.zsa, P.xsa, vzero, &tzc, &txc);
+            const double tauv = tv - tana(i - sgntz, j, dz, dx, P.zsa, P.xsa, vzero);
+            const double taue = te - tana(i, j - sgntx, dz, dx, P.zsa, P.xsa, vzero);
+            const double tauev = tev - tana(i - sgntz, j - sgntx, dz, dx, P.zsa, P.xsa, vzero);
+            t2d = delta(t2d, tauv, taue, tauev, t0c, tzc, txc, G.dzi, G.dxi, G.dz2i, G.dx2i,
+                        vzero, vref, sgntz, sgntx);
+            if (t2d < tv || t2d < te) t2d = kBig;
+        }
+    }
+    const double t0 = G.T(i, j);
+    const double tnew = pymin(pymin(t0, t1d), t2d);
+    G.T(i, j) = tnew;
+    if (tnew != t0) {
+        if (tnew == t1d1) G.setsgn(i, j, sgntz, 0);
+        else if (tnew == t1d2) G.setsgn(i, j, 0, sgntx);
+        else G.setsgn(i, j, sgntz, sgntx);
+        return true;
+    }
+    return false;
+}
+
+__global__ void fill2d_kernel(double *tt, long long n, uint8_t *sgn) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        tt[i] = kBig;
+        if (sgn) sgn[i] = 0;
+    }
+}
+
+// Source initialisation (_fteik2d.py:355-617) and the sweep sets, one lane per source.
+__global__ void fteik2d_kernel(const double *slow, int nz, int nx, double dz, double dx,
+                               const SrcParam *prm, int nsrc, int nsweep, int max_sets,
+                               double *tt_all, uint8_t *sgn_all, double *vzero_out,
+                               int *sets_done) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsrc) return;
+    const SrcParam P = prm[s];
+    Grid2 G;
+    G.slow = slow;
+    G.nz = nz;
+    G.nx = nx;
+    G.NZ = nz + 1;
+    G.NX = nx + 1;
+    const long long nn = (long long)G.NZ * G.NX;
+    G.tt = tt_all + (long long)s * nn;
+    G.sgn = sgn_all ? sgn_all + (long long)s * nn : nullptr;
+    G.dz = dz;
+    G.dx = dx;
+    const double vzero = slow[(long long)P.zsi * nx + P.xsi];
+    vzero_out[s] = vzero;
+    const int zsi = P.zsi, xsi = P.xsi, NZ = G.NZ, NX = G.NX;
+    const double zsa = P.zsa, xsa = P.xsa;
+
+    if (P.iflag == 2) {
+        const double dzu = fabs(zsa - (double)zsi), dzd = 1.0 - dzu;
+        const double dxw = fabs(xsa - (double)xsi), dxe = 1.0 - dxw;
+        for (int c = 0; c < 4; ++c) {  // corners (0,0) (1,0) (0,1) (1,1)
+            const int i = zsi + (c & 1), j = xsi + (c >> 1);
+            double tzc, txc;
+            G.T(i, j) = tanad(i, j, dz, dx, zsa, xsa, vzero, &tzc, &txc);
+        }
+        double dxi = 1.0 / dx, dx2i = dxi / dx, dzi, dz2i, tzc, txc, t0c;
+        // +x along rows zsi+1 and zsi (the 1D time td runs along row zsi's cells)
+        double td_prev = vzero * dxe * dx;
+        for (int j = xsi + 2; j < NX; ++j) {
+            const double vref = G.S(zsi, j - 1);
+            const double td = td_prev + dx * vref;
+            const double tauv = td - vzero * fabs((double)j - xsa) * dx;
+            const double tauev = td_prev - vzero * fabs((double)j - xsa - 1.0) * dx;
+            dzi = 1.0 / dzd;
+            dz2i = dz / dzd / dzd;
+            double taue = G.T(zsi + 1, j - 1) - tana(zsi + 1, j - 1, dz, dx, zsa, xsa, vzero);
+            t0c = tanad(zsi + 1, j, dz, dx, zsa, xsa, vzero, &tzc, &txc);
+            G.T(zsi + 1, j) = delta(G.T(zsi + 1, j), tauv, taue, tauev, t0c, tzc, txc, dzi, dxi,
+                                    dz2i, dx2i, vzero, vref, 1, 1);
+            G.setsgn(zsi + 1, j, 1, 1);
+            if (dzu > 0.0) {
+                dzi = 1.0 / dzu;
+                dz2i = dz / dzu / dzu;
+                taue = G.T(zsi, j - 1) - tana(zsi, j - 1, dz, dx, zsa, xsa, vzero);
+                t0c = tanad(zsi, j, dz, dx, zsa, xsa, vzero, &tzc, &txc);
+                G.T(zsi, j) = delta(G.T(zsi, j), tauv, taue, tauev, t0c, tzc, txc, dzi, dxi, dz2i,
+                                    dx2i, vzero, vref, -1, 1);
+                G.setsgn(zsi, j, -1, 1);
+            }
+            td_prev = td;
+        }
+        // -x
+        td_prev = vzero * dxw * dx;
+        for (int j = xsi - 1; j >= 0; --j) {
+            const double vref = G.S(zsi, j);
+            const double td = td_prev + dx * vref;
+            const double tauv = td - vzero * fabs((double)j - xsa) * dx;
+            const double tauev = td_prev - vzero * fabs((double)j - xsa + 1.0) * dx;
+            dzi = 1.0 / dzd;
+            dz2i = dz / dzd / dzd;
+            double taue = G.T(zsi + 1, j + 1) - tana(zsi + 1, j + 1, dz, dx, zsa, xsa, vzero);
+            t0c = tanad(zsi + 1, j, dz, dx, zsa, xsa, vzero, &tzc, &txc);
+            G.T(zsi + 1, j) = delta(G.T(zsi + 1, j), tauv, taue, tauev, t0c, tzc, txc, dzi, dxi,
+                                    dz2i, dx2i, vzero, vref, 1, -1);
+            G.setsgn(zsi + 1, j, 1, -1);
+            if (dzu > 0.0) {
+                dzi = 1.0 / dzu;
+                dz2i = dz / dzu / dzu;
+                // row zsi+1 is read here too (_fteik2d.py:472-474)
+                taue = G.T(zsi + 1, j + 1) - tana(zsi + 1, j + 1, dz, dx, zsa, xsa, vzero);
+                t0c = tanad(zsi, j, dz, dx, zsa, xsa, vzero, &tzc, &txc);
+                G.T(zsi, j) = delta(G.T(zsi, j), tauv, taue, tauev, t0c, tzc, txc, dzi, dxi, dz2i,
+                                    dx2i, vzero, vref, -1, -1);
+                G.setsgn(zsi, j, -1, -1);
+            }
+            td_prev = td;
+        }
+        // +z along columns xsi+1 and xsi
+        dzi = 1.0 / dz;
+        dz2i = dzi / dz;
+        td_prev = vzero * dzd * dz;
+        for (int i = zsi + 2; i < NZ; ++i) {
+            const double vref = G.S(i - 1, xsi);
+            const double td = td_prev + dz * vref;
+            const double taue = td - vzero * fabs((double)i - zsa) * dz;
+            const double tauev = td_prev - vzero * fabs((double)i - zsa - 1.0) * dz;
+            dxi = 1.0 / dxe;
+            dx2i = dx / dxe / dxe;
+            double tauv = G.T(i - 1, xsi + 1) - tana(i - 1, xsi + 1, dz, dx, zsa, xsa, vzero);
+            t0c = tanad(i, xsi + 1, dz, dx, zsa, xsa, vzero, &tzc, &txc);
+            G.T(i, xsi + 1) = delta(G.T(i, xsi + 1), tauv, taue, tauev, t0c, tzc, txc, dzi, dxi,
+                                    dz2i, dx2i, vzero, vref, 1, 1);
+            G.setsgn(i, xsi + 1, 1, 1);
+            if (dxw > 0.0) {
+                dxi = 1.0 / dxw;
+                dx2i = dx / dxw / dxw;
+                tauv = G.T(i - 1, xsi) - tana(i - 1, xsi, dz, dx, zsa, xsa, vzero);
+                t0c = tanad(i, xsi, dz, dx, zsa, xsa, vzero, &tzc, &txc);
+                G.T(i, xsi) = delta(G.T(i, xsi), tauv, taue, tauev, t0c, tzc, txc, dzi, dxi, dz2i,
+                                    dx2i, vzero, vref, 1, -1);
+                G.setsgn(i, xsi, 1, -1);
+            }
+            td_prev = td;
+        }
+        // -z
+        td_prev = vzero * dzu * dz;
+        for (int i = zsi - 1; i >= 0; --i) {
+            const double vref = G.S(i, xsi);
+            const double td = td_prev + dz * vref;
+            const double taue = td - vzero * fabs((double)i - zsa) * dz;
+            const double tauev = td_prev - vzero * fabs((double)i - zsa + 1.0) * dz;
+            dxi = 1.0 / dxe;
+            dx2i = dx / dxe / dxe;
+            double tauv = G.T(i + 1, xsi + 1) - tana(i + 1, xsi + 1, dz, dx, zsa, xsa, vzero);
+            t0c = tanad(i, xsi + 1, dz, dx, zsa, xsa, vzero, &tzc, &txc);
+            G.T(i, xsi + 1) = delta(G.T(i, xsi + 1), tauv, taue, tauev, t0c, tzc, txc, dzi, dxi,
+                                    dz2i, dx2i, vzero, vref, -1, 1);
+            G.setsgn(i, xsi + 1, -1, 1);
+            if (dxw > 0.0) {
+                dxi = 1.0 / dxw;
+                dx2i = dx / dxw / dxw;
+                tauv = G.T(i + 1, xsi) - tana(i + 1, xsi, dz, dx, zsa, xsa, vzero);
+                t0c = tanad(i, xsi, dz, dx, zsa, xsa, vzero, &tzc, &txc);
+                G.T(i, xsi) = delta(G.T(i, xsi), tauv, taue, tauev, t0c, tzc, txc, dzi, dxi, dz2i,
+                                    dx2i, vzero, vref, -1, -1);
+                G.setsgn(i, xsi, -1, -1);
+            }
+            td_prev = td;
+        }
+    } else {
+        G.T((int)zsa, (int)xsa) = 0.0;
+    }
+
+    // sweep sets, _fteik2d.py:190-287 and :619-621
+    G.dzi = 1.0 / dz;
+    G.dxi = 1.0 / dx;
+    G.dz2i = G.dzi / dz;
+    G.dx2i = G.dxi / dx;
+    const int cap = nsweep > 0 ? nsweep : max_sets;
+    int done = 0;
+    for (int set = 0; set < cap; ++set) {
+        bool ch = false;
+        for (int j = 1; j < NX; ++j) {
+            for (int i = 1; i < NZ; ++i) ch |= node2(G, P, vzero, i, j, 1, 1, 1, 1);
+            for (int i = NZ - 2; i >= 0; --i) ch |= node2(G, P, vzero, i, j, 0, 1, -1, 1);
+        }
+        for (int j = NX - 2; j >= 0; --j) {
+            for (int i = 1; i < NZ; ++i) ch |= node2(G, P, vzero, i, j, 1, 0, 1, -1);
+            for (int i = NZ - 2; i >= 0; --i) ch |= node2(G, P, vzero, i, j, 0, 0, -1, -1);
+        }
+        ++done;
+        // a set that changed nothing leaves every later set a no-op
+        if (!ch) break;
+    }
+    sets_done[s] = done;
+}
+
+// 2D gradient post-pass, _fteik2d.py:623-639 (seeded analytic components survive where
+// no upwind sign was recorded).
+__global__ void gradient2d_kernel(const double *tt, const uint8_t *sgn, int NZ, int NX, double dz,
+                                  double dx, SrcParam P, double vzero, double *g) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (long long)NZ * NX) return;
+    const int j = (int)(n % NX), i = (int)(n / NX);
+    double gz = 0.0, gx = 0.0;
+    if (P.iflag == 2 && i - P.zsi >= 0 && i - P.zsi <= 1 && j - P.xsi >= 0 && j - P.xsi <= 1)
+        tanad(i, j, dz, dx, P.zsa, P.xsa, vzero, &gz, &gx);
+    const uint8_t s = sgn[n];
+    const int sz = sgn_axis(s, 0), sx = sgn_axis(s, 1);
+    if (sz != 0) gz = sz * (tt[n] - tt[n - (long long)sz * NX]) / dz;
+    if (sx != 0) gx = sx * (tt[n] - tt[n - sx]) / dx;
+    const double gn = sqrt(gz * gz + gx * gx);
+    if (gn > 0.0) {
+        gz /= gn;
+        gx /= gn;
+    }
+    g[2 * n] = gz;
+    g[2 * n + 1] = gx;
+}
+
+// Host: bound check and source classification, _fteik2d.py:296-352.
+bool place_source2(const double *s, int nz, int nx, double dz, double dx, SrcParam *P) {
+    const double z = s[0], x = s[1];
+    if (!((0.0 <= z && z <= dz * nz) && (0.0 <= x && x <= dx * nx))) return false;
+    double zsa = z / dz, xsa = x / dx;
+    if (zsa >= nz) zsa = zsa - kEps;
+    if (xsa >= nx) xsa = xsa - kEps;
+    const int zsi = std::min((int)zsa, nz - 1), xsi = std::min((int)xsa, nx - 1);
+    const double dzu = std::fabs(zsa - (double)zsi), dzd = 1.0 - dzu;
+    const double dxw = std::fabs(xsa - (double)xsi), dxe = 1.0 - dxw;
+    const double dzv_min = pymin(dzu, dzd), dzh_min = pymin(dxw, dxe);
+    int iflag;
+    if (dzv_min < kEps && dzh_min < kEps) {
+        zsa = std::nearbyint(zsa);
+        xsa = std::nearbyint(xsa);
+        iflag = 1;
+    } else if (dzv_min > kEps || dzh_min > kEps) {
+        zsa = dzv_min < kEps ? std::nearbyint(zsa) : zsa;
+        xsa = dzh_min < kEps ? std::nearbyint(xsa) : xsa;
+        iflag = 2;
+    } else {
+        zsa = std::nearbyint(zsa);
+        xsa = std::nearbyint(xsa);
+        iflag = 3;
+    }
+    P->zsa = zsa;
+    P->xsa = xsa;
+    P->ysa = 0.0;
+    P->zsi = zsi;
+    P->xsi = xsi;
+    P->ysi = 0;
+    P->iflag = iflag;
+    return true;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+}  // namespace
+}  // namespace fteik
+
+using namespace fteik;
+
+extern "C" size_t fteik2d_workspace_size(int nsrc) { return align256(sizeof(SrcParam) * nsrc) + 256; }
+
+extern "C" int fteik2d_solve(const double *slow, int nz, int nx, double dz, double dx,
+                             const double *src, int nsrc, int nsweep, int max_sets, int want_grad,
+                             double *tt, uint8_t *sgn, double *vzero, int *sets_done,
+                             void *workspace, size_t workspace_bytes, void *stream_) {
+    if (nz < 1 || nx < 1 || nsrc < 1 || !slow || !src || !tt || !vzero || !sets_done ||
+        !workspace || !(dz > 0.0) || !(dx > 0.0))
+        return FTEIK_ERR_ARG;
+    if (want_grad && !sgn) return FTEIK_ERR_ARG;
+    if (nsweep <= 0 && max_sets < 1) return FTEIK_ERR_ARG;
+    if (workspace_bytes < fteik2d_workspace_size(nsrc)) return FTEIK_ERR_WORKSPACE;
+    hipStream_t stream = (hipStream_t)stream_;
+    std::vector<SrcParam> prm(nsrc);
+    for (int s = 0; s < nsrc; ++s)
+        if (!place_source2(src + 2 * s, nz, nx, dz, dx, &prm[s])) return FTEIK_ERR_SOURCE_OOB;
+    SrcParam *dprm = (SrcParam *)workspace;
+    FTEIK_CHECK(hipMemcpyAsync(dprm, prm.data(), sizeof(SrcParam) * nsrc, hipMemcpyHostToDevice,
+                               stream));
+    // the host vector goes out of scope on return: finish the copy first
+    FTEIK_CHECK(hipStreamSynchronize(stream));
+    const long long total = (long long)(nz + 1) * (nx + 1) * nsrc;
+    const int blocks = (int)std::min<long long>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(fill2d_kernel, dim3(blocks), dim3(256), 0, stream, tt, total,
+                       want_grad ? sgn : (uint8_t *)nullptr);
+    FTEIK_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(fteik2d_kernel, dim3((nsrc + 63) / 64), dim3(64), 0, stream, slow, nz, nx,
+                       dz, dx, dprm, nsrc, nsweep, max_sets, tt, want_grad ? sgn : nullptr, vzero,
+                       sets_done);
+    return hip_status(hipGetLastError());
+}
+
+extern "C" int fteik2d_gradient(const double *tt, const uint8_t *sgn, int nz, int nx, double dz,
+                                double dx, const double *src, double vzero, double *ttgrad,
+                                void *stream) {
+    if (!tt || !sgn || !ttgrad || !src || nz < 1 || nx < 1) return FTEIK_ERR_ARG;
+    SrcParam P;
+    if (!place_source2(src, nz, nx, dz, dx, &P)) return FTEIK_ERR_SOURCE_OOB;
+    const long long nn = (long long)(nz + 1) * (nx + 1);
+    hipLaunchKernelGGL(gradient2d_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, tt, sgn, nz + 1, nx + 1, dz, dx, P, vzero, ttgrad);
+    return hip_status(hipGetLastError());
+}

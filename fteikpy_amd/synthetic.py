@@ -1,0 +1,82 @@
+"""Deterministic synthetic velocity models and acquisition geometries for the benchmark
+workloads (numpy only, so every machine regenerates them bit for bit).
+
+- ``model_2d_gaussians``: 2D gradient medium with 4 Gaussian anomalies (101 x 301, 10 m)
+- ``model_3d_layers``: 3D layered medium with a smooth random perturbation (200^3, 5 m)
+- ``model_3d_inclusions``: 3D linear-gradient medium with 20 spherical inclusions
+  (160 x 320 x 320, 12.5 m) with a 16 x 16 surface source grid and 64 x 64 receivers
+"""
+import numpy as np
+
+
+def gaussian_filter_reflect(p, sigma):
+    """Separable Gaussian filter (truncate at 4 sigma, reflected edges), FP64."""
+    r = int(4.0 * sigma + 0.5)
+    x = np.arange(-r, r + 1, dtype=np.float64)
+    w = np.exp(-0.5 * (x / sigma) ** 2)
+    w /= w.sum()
+    out = p
+    for ax in range(p.ndim):
+        pad = [(0, 0)] * p.ndim
+        pad[ax] = (r, r)
+        q = np.pad(out, pad, mode="symmetric")
+        acc = np.zeros_like(out)
+        n = out.shape[ax]
+        for t, wt in enumerate(w):
+            sl = [slice(None)] * p.ndim
+            sl[ax] = slice(t, t + n)
+            acc += wt * q[tuple(sl)]
+        out = acc
+    return out
+
+
+def model_2d_gaussians(seed=1001):
+    """v(z,x) = 1500 + 0.6 z plus 4 Gaussian anomalies of +-300 m/s, sigma 150 m."""
+    rng = np.random.default_rng(seed)
+    nz, nx, d = 101, 301, 10.0
+    z = np.arange(nz)[:, None] * d
+    x = np.arange(nx)[None, :] * d
+    v = 1500.0 + 0.6 * z + 0.0 * x
+    cen = rng.uniform([0.0, 0.0], [1000.0, 3000.0], size=(4, 2))
+    for amp, (cz, cx) in zip((300.0, -300.0, 300.0, -300.0), cen):
+        v = v + amp * np.exp(-((z - cz) ** 2 + (x - cx) ** 2) / (2.0 * 150.0 ** 2))
+    src = np.array([12.3, 1013.7])
+    rec = rng.uniform([0.0, 0.0], [1010.0, 3010.0], size=(200, 2))
+    return v, (d, d), src, rec
+
+
+def model_3d_layers(seed=1002, n=200, d=5.0, perturb_sigma=8.0):
+    """5 equal horizontal layers of 1800..4500 m/s times (1 + p), max |p| = 5%."""
+    rng = np.random.default_rng(seed)
+    layers = np.array([1800.0, 2475.0, 3150.0, 3825.0, 4500.0])
+    v = np.repeat(layers, int(np.ceil(n / 5)))[:n][:, None, None] * np.ones((n, n, n))
+    p = gaussian_filter_reflect(rng.standard_normal((n, n, n)), perturb_sigma)
+    v = v * (1.0 + 0.05 * p / np.abs(p).max())
+    src = np.array([502.3, 497.1, 503.9]) * (n * d / 1000.0)
+    rec = rng.uniform(0.0, n * d, size=(10000, 3))
+    return v, (d, d, d), src, rec
+
+
+def model_3d_inclusions(seed=1003, shape=(160, 320, 320), d=12.5, nsrc_side=16, nrec_side=64):
+    """v = 2000 + 0.8 z times (1 +- 0.15) inside 20 spheres (radius 4..16 cells, sign
+    alternating); sources on a 16 x 16 grid at z = d/2, receivers on a 64 x 64 grid at z = 0."""
+    rng = np.random.default_rng(seed)
+    nz, nx, ny = shape
+    z = np.arange(nz)[:, None, None] * d
+    v = (2000.0 + 0.8 * z) * np.ones(shape)
+    iz = np.arange(nz)[:, None, None]
+    ix = np.arange(nx)[None, :, None]
+    iy = np.arange(ny)[None, None, :]
+    for s in range(20):
+        c = rng.uniform([0, 0, 0], [nz, nx, ny])
+        r = rng.uniform(4.0, 16.0)
+        inside = (iz - c[0]) ** 2 + (ix - c[1]) ** 2 + (iy - c[2]) ** 2 <= r * r
+        v[inside] *= 1.15 if s % 2 == 0 else 0.85
+    ex, ey = nx * d, ny * d
+    sx = (np.arange(nsrc_side) + 0.5) * ex / nsrc_side
+    sy = (np.arange(nsrc_side) + 0.5) * ey / nsrc_side
+    src = np.stack(np.meshgrid(np.full(1, d / 2), sx, sy, indexing="ij"), -1).reshape(-1, 3)
+    rx = (np.arange(nrec_side) + 0.5) * ex / nrec_side
+    ry = (np.arange(nrec_side) + 0.5) * ey / nrec_side
+    rec = np.stack(np.meshgrid(np.zeros(1), rx, ry, indexing="ij"), -1).reshape(-1, 3)
+    return v, (d, d, d), src, rec

@@ -1,0 +1,163 @@
+/*
+ * fteik_amd: C ABI of the MI355X (gfx950) fast-sweeping eikonal library.
+ *
+ * Every entry point takes plain pointers and sizes, no C++ or torch types.  Device
+ * pointers are HIP device memory (e.g. torch tensors' data_ptr()); "host" pointers are
+ * ordinary host memory read before the call returns.  `stream` is a hipStream_t (NULL =
+ * the default stream); every call is stream-ordered and only enqueues work unless stated.
+ *
+ * The reference (fteikpy 2.4.0) is pure Python/numba with no FFI; each entry point below
+ * replaces one numba-compiled function called from the reference's Python layer, cited
+ * as file:line of fteikpy/.  Array conventions are the reference's: slowness is cell based
+ * (nz, nx[, ny]) C-ordered FP64, traveltime is node based (nz+1, nx+1[, ny+1]).
+ *
+ * Return codes: 0 OK; 1 source out of bound; 2 end point out of bound; 3 maximum number
+ * of steps reached; 4 invalid argument; 5 workspace too small; negative: -(hipError_t).
+ */
+#ifndef FTEIK_AMD_H
+#define FTEIK_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FTEIK_OK 0
+#define FTEIK_ERR_SOURCE_OOB 1
+#define FTEIK_ERR_ENDPOINT_OOB 2
+#define FTEIK_ERR_MAX_STEP 3
+#define FTEIK_ERR_ARG 4
+#define FTEIK_ERR_WORKSPACE 5
+
+/* Library version string. */
+const char *fteik_version(void);
+
+/* ---------------------------------------------------------------------------------- */
+/* 3D solve                                                                            */
+/* ---------------------------------------------------------------------------------- */
+
+/* Device workspace (bytes) fteik3d_solve needs for nsrc sources. */
+size_t fteik3d_workspace_size(int nsrc);
+
+/*
+ * Replaces solve3d / fteik3d / fteik3d_vectorized (fteikpy/_fteik/_fteik3d.py:394-525),
+ * called from Eikonal3D.solve (fteikpy/_solver.py:120-126).
+ *
+ * slow      device [nz, nx, ny] slowness (1/velocity), shared by all sources
+ * src       host   [nsrc, 3] source coordinates relative to the grid origin
+ * nsweep    > 0: exactly nsweep sweep sets (the reference's nsweep);
+ *           <= 0: sets until one changes no node, at most max_sets
+ * want_grad record the upwind direction of every update (needed by fteik3d_gradient)
+ * tile_a    nodes per brick along z (0 = library default for the shape)
+ * tt        device [nsrc, nz+1, nx+1, ny+1] traveltimes (output)
+ * sgn       device [nsrc, nz+1, nx+1, ny+1] uint8 packed upwind signs or NULL
+ * vzero     device [nsrc] slowness of each source cell (output)
+ * sets_done device [nsrc] sweep sets executed per source (output)
+ * workspace device, at least fteik3d_workspace_size(nsrc) bytes
+ * use_graph capture one sweep set as a hipGraph and replay it (1) or launch directly (0)
+ * Checks all sources before enqueuing anything: returns 1 ("source out of bound") with no
+ * partial output, as the reference raises (fteikpy/_fteik/_fteik3d.py:402-407).
+ * In converge mode (nsweep <= 0) the call synchronizes the stream once per set.
+ */
+int fteik3d_solve(const double *slow, int nz, int nx, int ny, double dz, double dx, double dy,
+                  const double *src, int nsrc, int nsweep, int max_sets, int want_grad,
+                  int tile_a, double *tt, uint8_t *sgn, double *vzero, int *sets_done,
+                  void *workspace, size_t workspace_bytes, int use_graph, void *stream);
+
+/*
+ * Replaces the gradient post-pass of fteik3d (fteikpy/_fteik/_fteik3d.py:464-488) that
+ * fills TraveltimeGrid3D.gradient (fteikpy/_grid.py:416-427).
+ * tt, sgn: device, one source ([nz+1, nx+1, ny+1]); vzero: that source's value;
+ * src: host [3] relative to the origin.  ttgrad: device [nz+1, nx+1, ny+1, 3] (output).
+ */
+int fteik3d_gradient(const double *tt, const uint8_t *sgn, int nz, int nx, int ny, double dz,
+                     double dx, double dy, const double *src, double vzero, double *ttgrad,
+                     void *stream);
+
+/* ---------------------------------------------------------------------------------- */
+/* 2D solve                                                                            */
+/* ---------------------------------------------------------------------------------- */
+
+size_t fteik2d_workspace_size(int nsrc);
+
+/*
+ * Replaces solve2d / fteik2d / fteik2d_vectorized (fteikpy/_fteik/_fteik2d.py:290-674),
+ * called from Eikonal2D.solve (fteikpy/_solver.py:48-54).  Same conventions as
+ * fteik3d_solve with src [nsrc, 2] and tt [nsrc, nz+1, nx+1].  The reference's 2D set is
+ * a single serial dependency chain (columns interleave the +z and -z passes), so each
+ * source is swept in the reference's exact order by one GPU lane; sources run in parallel.
+ */
+int fteik2d_solve(const double *slow, int nz, int nx, double dz, double dx, const double *src,
+                  int nsrc, int nsweep, int max_sets, int want_grad, double *tt, uint8_t *sgn,
+                  double *vzero, int *sets_done, void *workspace, size_t workspace_bytes,
+                  void *stream);
+
+/* Replaces the 2D gradient post-pass (fteikpy/_fteik/_fteik2d.py:623-639). */
+int fteik2d_gradient(const double *tt, const uint8_t *sgn, int nz, int nx, double dz, double dx,
+                     const double *src, double vzero, double *ttgrad, void *stream);
+
+/* ---------------------------------------------------------------------------------- */
+/* Interpolation                                                                       */
+/* ---------------------------------------------------------------------------------- */
+
+/*
+ * Trilinear interpolation, replaces interp3d (fteikpy/_interp/_interp3d.py:7-195), called
+ * from BaseGrid3D.__call__ (fteikpy/_base.py:139-163).
+ * v: device grid [n0, n1, n2] read with element stride `stride` (1 for a plain grid, 3 for
+ * one component of an interleaved gradient grid).  origin, d: host [3].  q: device [nq, 3].
+ * out: device [nq]; fill for points outside the grid.
+ */
+int fteik3d_interp(const double *v, int n0, int n1, int n2, int stride, const double *origin,
+                   const double *d, const double *q, int nq, double fill, double *out,
+                   void *stream);
+
+/*
+ * Apparent-velocity trilinear traveltime interpolation, replaces vinterp3d
+ * (fteikpy/_interp/_vinterp3d.py:7-291), called from TraveltimeGrid3D.__call__
+ * (fteikpy/_grid.py:342-368).  tt: device [n0, n1, n2]; src: host [3] ABSOLUTE source
+ * coordinates; vzero: slowness of the source cell.
+ */
+int fteik3d_vinterp(const double *tt, int n0, int n1, int n2, const double *origin,
+                    const double *d, const double *q, int nq, const double *src, double vzero,
+                    double fill, double *out, void *stream);
+
+/* 2D twins: interp2d (fteikpy/_interp/_interp2d.py:7-95), vinterp2d
+ * (fteikpy/_interp/_vinterp2d.py:7-131).  origin, d, src: host [2]; q: device [nq, 2]. */
+int fteik2d_interp(const double *v, int n0, int n1, int stride, const double *origin,
+                   const double *d, const double *q, int nq, double fill, double *out,
+                   void *stream);
+int fteik2d_vinterp(const double *tt, int n0, int n1, const double *origin, const double *d,
+                    const double *q, int nq, const double *src, double vzero, double fill,
+                    double *out, void *stream);
+
+/* ---------------------------------------------------------------------------------- */
+/* Ray tracing                                                                         */
+/* ---------------------------------------------------------------------------------- */
+
+/*
+ * A-posteriori ray tracing, replaces ray3d (fteikpy/_fteik/_ray3d.py:9-208), called from
+ * TraveltimeGrid3D.raytrace (fteikpy/_grid.py:370-414).
+ * grad: device [n0, n1, n2, 3] normalized gradient (fteik3d_gradient output).
+ * p: device [n, 3] end points (receivers), src: host [3] ABSOLUTE source.
+ * rays: device [n, max_step, 3] work buffer in receiver-to-source order (as the
+ * reference's buffer); counts: device [n] index of the last point; status: device [n]
+ * per-ray code (0, 2 end point out of bound, 3 maximum number of steps reached).
+ * Enqueues only: the caller reads status after synchronizing.
+ */
+int fteik3d_raytrace(const double *grad, int n0, int n1, int n2, const double *origin,
+                     const double *d, const double *p, int n, const double *src,
+                     double stepsize, int max_step, int honor_grid, double *rays, int *counts,
+                     int *status, void *stream);
+
+/* 2D twin of fteik3d_raytrace, replaces ray2d (fteikpy/_fteik/_ray2d.py:9-153). */
+int fteik2d_raytrace(const double *grad, int n0, int n1, const double *origin, const double *d,
+                     const double *p, int n, const double *src, double stepsize, int max_step,
+                     int honor_grid, double *rays, int *counts, int *status, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FTEIK_AMD_H */

@@ -1,0 +1,89 @@
+"""CPU checks of the C ABI: the library loads and exports every declared entry point, and
+the host-side argument checks reject bad input without touching a GPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "fteik_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fteik\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from fteikpy_amd import _lib
+
+    L = _lib.lib()
+    names = _declared()
+    assert names, "no declarations found"
+    for name in names:
+        assert hasattr(L, name), name
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+    assert set(_lib.SIGNATURES) == set(names)
+    assert b"gfx950" in L.fteik_version()
+
+
+def test_workspace_sizes():
+    from fteikpy_amd import _lib
+
+    L = _lib.lib()
+    assert L.fteik3d_workspace_size(1) > 0
+    assert L.fteik3d_workspace_size(256) > L.fteik3d_workspace_size(1)
+    assert L.fteik2d_workspace_size(4) > 0
+
+
+def test_host_argument_checks():
+    """Calls that fail validation return before any HIP call (no device needed)."""
+    from fteikpy_amd import _lib
+
+    L = _lib.lib()
+    src = np.zeros(3)
+    sp = src.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    # null slowness pointer
+    st = L.fteik3d_solve(None, 4, 4, 4, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, None, None, None,
+                         None, None, 0, 0, None)
+    assert st == _lib.ERR_ARG
+    with pytest.raises(ValueError, match="invalid argument"):
+        _lib.check(st)
+    with pytest.raises(ValueError, match="source out of bound"):
+        _lib.check(_lib.ERR_SOURCE_OOB)
+    with pytest.raises(RuntimeError, match="maximum number of steps"):
+        _lib.check(_lib.ERR_MAX_STEP)
+
+
+def test_source_check_before_any_device_work():
+    """An out-of-bound source is rejected from the host (the reference raises before
+    allocating, fteikpy/_fteik/_fteik3d.py:402-407): fake non-null device pointers are
+    never dereferenced."""
+    from fteikpy_amd import _lib
+
+    L = _lib.lib()
+    fake = ctypes.c_void_p(0x1000)
+    src = np.array([-1.0, 0.0, 0.0])
+    sp = src.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    ws = L.fteik3d_workspace_size(1)
+    st = L.fteik3d_solve(fake, 4, 4, 4, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, fake, None, fake, fake,
+                         fake, ws, 0, None)
+    assert st == _lib.ERR_SOURCE_OOB
+    src2 = np.array([0.0, 4.5])
+    sp2 = src2.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    st = L.fteik2d_solve(fake, 4, 4, 1.0, 1.0, sp2, 1, 2, 10, 0, fake, None, fake, fake, fake,
+                         L.fteik2d_workspace_size(1), None)
+    assert st == _lib.ERR_SOURCE_OOB
+
+
+def test_product_does_not_import_oracle():
+    """The product package must not reach the checker."""
+    pkg = os.path.join(ROOT, "fteikpy_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in text and "from oracle" not in text, f
+                assert "libfteik_oracle" not in text, f

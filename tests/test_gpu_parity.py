@@ -1,0 +1,191 @@
+"""GPU parity: the HIP path against the oracle (itself pinned to the reference in
+test_oracle.py) on the same inputs.
+
+Bar: traveltimes are bit-identical to the oracle at every nsweep (each directional pass
+is order-invariant and the library is built without FP contraction), and hence within
+1e-15 of the reference's golden vectors; the number of sets to convergence matches;
+gradients, interpolated values and ray points within 1e-12 relative (the reference bar
+of the north star is 1e-9 / 1e-8).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def fa(device):
+    import fteikpy_amd
+
+    return fteikpy_amd
+
+
+def _gold(dim):
+    return np.load(os.path.join(GOLD, f"golden_{dim}d.npz"))
+
+
+def _tags(g, s):
+    k = int(g[f"s{s}_kconv"])
+    return [(ns, f"s{s}_n{ns}" if ns != k else f"s{s}_conv") for ns in sorted({1, 2, 3, k})]
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_golden_solve(fa, dim):
+    g = _gold(dim)
+    v, d, o = g["v"], g["gridsize"], g["origin"]
+    Eik = fa.Eikonal3D if dim == 3 else fa.Eikonal2D
+    solve = oracle.solve3d if dim == 3 else oracle.solve2d
+    eik = Eik(v, d, o)
+    for s, src in enumerate(g["sources"]):
+        print(f"golden {dim}D source {s}", flush=True)
+        for ns, tag in _tags(g, s):
+            tt = eik.solve(src, nsweep=ns, return_gradient=True)
+            ref, rgrad, rvz, _ = solve(1.0 / v, *d, src - o, nsweep=ns, grad=True)
+            assert np.array_equal(tt.grid, ref), (s, ns)
+            np.testing.assert_allclose(tt.grid, g[tag + "_tt"], rtol=1e-15, atol=0)
+            assert tt.vzero == rvz == float(g[tag + "_vzero"])
+            grad = np.stack([c.grid for c in tt.gradient], -1)
+            np.testing.assert_allclose(grad, rgrad, rtol=1e-13, atol=1e-15)
+            np.testing.assert_allclose(grad, g[tag + "_grad"], rtol=1e-12, atol=1e-14)
+        conv = eik.solve(src, nsweep=None)
+        assert conv.sets_done == int(g[f"s{s}_kconv"])
+        assert np.array_equal(conv.grid, g[f"s{s}_conv_tt"]) or np.allclose(
+            conv.grid, g[f"s{s}_conv_tt"], rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_golden_interp_and_rays(fa, dim):
+    g = _gold(dim)
+    v, d, o = g["v"], g["gridsize"], g["origin"]
+    Eik = fa.Eikonal3D if dim == 3 else fa.Eikonal2D
+    eik = Eik(v, d, o)
+    np.testing.assert_array_equal(eik(g["vq"]), g["v_interp"])
+    for s, src in enumerate(g["sources"]):
+        tt = eik.solve(src, nsweep=int(g[f"s{s}_kconv"]), return_gradient=True)
+        np.testing.assert_allclose(tt(g[f"s{s}_q"]), g[f"s{s}_vinterp"], rtol=1e-13, atol=0)
+        np.testing.assert_allclose(tt.gradient[0](g[f"s{s}_q"]), g[f"s{s}_gz_interp"], rtol=1e-12,
+                                   atol=1e-14)
+        for hg in (0, 1):
+            rays = tt.raytrace(g[f"s{s}_rec"], honor_grid=bool(hg))
+            assert [len(r) for r in rays] == list(g[f"s{s}_rays_hg{hg}_len"]), (s, hg)
+            np.testing.assert_allclose(np.concatenate(rays), g[f"s{s}_rays_hg{hg}"], rtol=1e-12,
+                                       atol=1e-9)
+
+
+def test_batch_list_and_single_point_returns(fa):
+    g = _gold(3)
+    eik = fa.Eikonal3D(g["v"], g["gridsize"], g["origin"])
+    tts = eik.solve(g["sources"][:2], nsweep=2)
+    assert isinstance(tts, list) and len(tts) == 2
+    np.testing.assert_allclose(np.stack([t.grid for t in tts]), g["batch_tt"], rtol=1e-15, atol=0)
+    assert isinstance(tts[0](g["sources"][0]), float)
+    assert np.isnan(tts[0](g["origin"] - 1.0))
+
+
+def _rough_model(shape, seed):
+    rng = np.random.default_rng(seed)
+    p = rng.standard_normal(shape)
+    for ax in range(len(shape)):
+        p = (np.roll(p, 1, ax) + p + np.roll(p, -1, ax)) / 3.0
+    return 1500.0 * np.exp(0.8 * p / np.abs(p).max())
+
+
+@pytest.mark.parametrize("tile_a", [1, 3, 8, 0, 64])
+def test_brick_shapes_bitwise(fa, tile_a):
+    """Partial bricks in every axis, several z brick heights, 3 sources in one launch."""
+    shape, d = (21, 19, 27), (7.0, 9.0, 8.0)
+    v = _rough_model(shape, 11)
+    srcs = np.array([[3.3, 17.7, 40.1], [140.0, 60.0, 100.0], [147.0, 171.0, 216.0]])
+    ref, _, _, kref = oracle.solve3d(1.0 / v, *d, srcs, nsweep=0)
+    tt, _, _, sets = fa.Eikonal3D(v, d).solve_tensors(srcs, nsweep=None, tile_a=tile_a)
+    assert list(sets) == list(kref)
+    assert np.array_equal(tt.cpu().numpy(), ref)
+
+
+def test_graph_mode_matches_direct(fa):
+    shape, d = (18, 22, 25), (10.0, 10.0, 10.0)
+    v = _rough_model(shape, 3)
+    srcs = np.array([[5.0, 5.0, 5.0], [170.0, 200.0, 240.0]])
+    eik = fa.Eikonal3D(v, d)
+    a = eik.solve_tensors(srcs, nsweep=None, use_graph=False)
+    for _ in range(2):  # replays with eager work between them
+        b = eik.solve_tensors(srcs, nsweep=None, use_graph=True)
+        assert np.array_equal(a[0].cpu().numpy(), b[0].cpu().numpy())
+        assert list(a[3]) == list(b[3])
+        b2 = eik.solve_tensors(srcs, nsweep=3, use_graph=True)
+        a2 = eik.solve_tensors(srcs, nsweep=3, use_graph=False)
+        assert np.array_equal(a2[0].cpu().numpy(), b2[0].cpu().numpy())
+
+
+def test_larger_model_bitwise(fa):
+    shape, d = (48, 56, 64), (10.0, 10.0, 10.0)
+    v = _rough_model(shape, 5)
+    src = np.array([123.4, 301.7, 222.2])
+    ref, rgrad, _, k = oracle.solve3d(1.0 / v, *d, src, nsweep=0, grad=True)
+    tt = fa.Eikonal3D(v, d).solve(src, nsweep=None, return_gradient=True)
+    assert tt.sets_done == k
+    assert np.array_equal(tt.grid, ref)
+    grad = np.stack([c.grid for c in tt.gradient], -1)
+    np.testing.assert_allclose(grad, rgrad, rtol=1e-13, atol=1e-15)
+
+
+def test_reference_pins(fa):
+    """Values of the reference's own tests (tests/test_eikonal{2,3}d.py, test_traveltime*.py,
+    test_gradient*.py)."""
+    e3 = fa.Eikonal3D(np.ones((16, 16, 16)), (1.0, 1.0, 1.0))
+    e2 = fa.Eikonal2D(np.ones((16, 16)), (1.0, 1.0))
+    for src, ref in (([0.0, 0.0, 0.0], 76631.78464590), ([3.5, 3.5, 3.5], 54091.60683054)):
+        assert np.allclose(e3.solve(src, nsweep=3).grid.sum(), ref)
+    for src, ref in (([0.0, 0.0], 3573.41870888), ([3.5, 3.5], 2462.96106736)):
+        assert np.allclose(e2.solve(src, nsweep=2).grid.sum(), ref)
+    tt = e3.solve([0.0, 0.0, 0.0], nsweep=3, return_gradient=True)
+    assert np.allclose(tt([1.5, 1.5, 1.5]), 2.60631816)
+    assert np.allclose(tt([15.0, 15.0, 15.0]), 25.98076211)
+    assert np.allclose(tt.gradient[0]([15.0, 15.0, 15.0]), 0.57735027)
+    assert np.allclose(tt.raytrace([1.5, 1.5, 1.5], stepsize=1.0).sum(), 8.30384757)
+    assert np.allclose(tt.raytrace([1.5, 1.5, 1.5], stepsize=1.0, honor_grid=True).sum(), 7.5)
+    tt2 = e2.solve([0.0, 0.0], nsweep=2, return_gradient=True)
+    assert np.allclose(tt2.raytrace([1.5, 1.5], stepsize=1.0).sum(), 4.75735931)
+    assert np.allclose(tt2.raytrace([1.5, 1.5], stepsize=1.0, honor_grid=True).sum(), 5.0)
+    assert np.allclose(tt2([0.5, 0.5]), 0.70710678)
+    assert np.allclose(e2.solve([3.5, 3.5], nsweep=2, return_gradient=True).gradient[0]([0.0, 0.0]),
+                       -0.70710678)
+    # meshio-test sums and ray lengths (tests/test_meshio.py)
+    t = fa.Eikonal3D(np.ones((8, 10, 12)), (1.0, 1.0, 1.0)).solve((4.0, 5.0, 6.0), return_gradient=True)
+    assert np.allclose(t.grid.sum(), 6909.90160991)
+    assert len(t.raytrace((0.0, 0.0, 0.0), honor_grid=True)) == 14
+    t = fa.Eikonal2D(np.ones((8, 10)), (1.0, 1.0)).solve((4.0, 5.0), return_gradient=True)
+    assert np.allclose(t.grid.sum(), 378.23469225)
+    assert len(t.raytrace((0.0, 0.0), honor_grid=True)) == 9
+
+
+def test_errors(fa):
+    e3 = fa.Eikonal3D(np.ones((8, 8, 8)), (1.0, 1.0, 1.0))
+    with pytest.raises(ValueError, match="source out of bound"):
+        e3.solve([9.0, 0.0, 0.0])
+    with pytest.raises(ValueError, match="source out of bound"):
+        fa.Eikonal2D(np.ones((8, 8)), (1.0, 1.0)).solve([0.0, -0.1])
+    tt = e3.solve([0.0, 0.0, 0.0], nsweep=2)
+    with pytest.raises(ValueError, match="no gradient grid"):
+        tt.gradient
+    tt = e3.solve([0.0, 0.0, 0.0], nsweep=2, return_gradient=True)
+    with pytest.raises(ValueError, match="end point out of bound"):
+        tt.raytrace([9.5, 0.0, 0.0])
+    with pytest.raises(RuntimeError, match="maximum number of steps"):
+        tt.raytrace([8.0, 8.0, 8.0], stepsize=1.0, max_step=2)
+
+
+def test_2d_many_sources(fa):
+    g = _gold(2)
+    v, d, o = g["v"], g["gridsize"], g["origin"]
+    rng = np.random.default_rng(2)
+    srcs = o + rng.random((37, 2)) * (np.array(v.shape) * d)
+    ref = oracle.solve2d(1.0 / v, *d, srcs - o, nsweep=3)[0]
+    tts = fa.Eikonal2D(v, d, o).solve(srcs, nsweep=3)
+    assert np.array_equal(np.stack([t.grid for t in tts]), ref)
