@@ -27,16 +27,19 @@ constexpr int kLanesB = 8;      // brick width along x (oriented b)
 constexpr int kLanesC = 8;      // brick width along y (oriented c)
 constexpr int kSkew = kLanesB + kLanesC - 2;  // steps between first and last lane = 14
 constexpr int kPlane = (kLanesB + 1) * (kLanesC + 1);  // 81 entries: node or cell block
-constexpr int kSlot = 2 * kPlane;  // tt block then slowness block
-constexpr int kPrefetch = 2;    // planes loaded ahead of lane (0,0)
-constexpr int kRing = kPrefetch + kSkew + 1;  // slots needed so a slot is free when reloaded
+constexpr int kPrefetch = 1;    // planes resident in LDS ahead of lane (0,0)
+constexpr int kDepth = 4;       // further planes in flight in registers (loop unrolled by it)
+// a slot is rewritten at the end of step s for plane s+kPrefetch; its previous plane was
+// last read (as the upwind plane of lane (7,7)) at step s-1, before that step's barrier
+constexpr int kRing = kPrefetch + kSkew + 2;
 
 struct SweepArgs {
     const double *slow;
     double *tt;
     uint8_t *sgn;
     int *changed;
-    const int *active;
+    const int *list;        // sources still changing (compacted), first *nlist entries
+    const int *nlist;
     long long node_stride;  // nodes per source grid
     int nz, nx, ny;         // cells
     int NZ, NX, NY;         // nodes
@@ -131,10 +134,17 @@ __device__ __forceinline__ double node_update(const SweepArgs &A, double t0, dou
     return tnew;
 }
 
-template <bool GRAD>
-__global__ __launch_bounds__(64) void sweep3d_brick_kernel(const SweepArgs A) {
-    const int src = blockIdx.y;
-    if (!A.active[src]) return;
+// Each workgroup runs W waves on the same brick for W different sources (from the
+// compacted list of sources still changing): the slowness block of every staged plane is
+// loaded once into LDS and shared, each wave owns the traveltime block of its source.
+template <bool GRAD, int W>
+__global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A) {
+    constexpr int kEnt = kPlane * (W + 1);  // staged entries per plane: W node blocks + cells
+    constexpr int kThreads = 64 * W;
+    constexpr int NE = (kEnt + kThreads - 1) / kThreads;
+    const int nact = *A.nlist;
+    const int g0 = blockIdx.y * W;
+    if (g0 >= nact) return;  // block-uniform: every source of this group has converged
 
     // brick (BA, BB, BC) on anti-diagonal A.diag, enumerated with BA outermost
     int t = blockIdx.x, BA = 0, BB = 0, BC = 0;
@@ -156,12 +166,14 @@ __global__ __launch_bounds__(64) void sweep3d_brick_kernel(const SweepArgs A) {
         }
     }
 
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const bool has_src = g0 + wave < nact;
+    const int src = has_src ? A.list[g0 + wave] : 0;
     const int bb = lane >> 3, cc = lane & 7;
-    const int a0 = 1 + BA * A.tile_a;          // first oriented plane of the brick
-    const int na = min(A.tile_a, A.NZ - a0);    // planes a0 .. a0+na-1
+    const int a0 = 1 + BA * A.tile_a;        // first oriented plane of the brick
+    const int na = min(A.tile_a, A.NZ - a0);  // planes a0 .. a0+na-1
     const int b0 = 1 + BB * kLanesB, c0 = 1 + BC * kLanesC;
-    const bool inb = (b0 + bb <= A.NX - 1) && (c0 + cc <= A.NY - 1);
+    const bool inb = has_src && (b0 + bb <= A.NX - 1) && (c0 + cc <= A.NY - 1);
 
     // oriented -> memory index maps
     const int NX = A.NX, NY = A.NY;
@@ -177,116 +189,126 @@ __global__ __launch_bounds__(64) void sweep3d_brick_kernel(const SweepArgs A) {
     const long long plane_nodes = (long long)NX * NY;
     const long long plane_cells = (long long)A.nx * A.ny;
 
-    // staging entries of this lane: e = lane + 64 n; e < 81 node block, 81..161 cell block
-    long long ent_off[3];
-    int ent_kind[3];  // 0 unused, 1 node in range, 2 node out of range, 3 cell
-    for (int n = 0; n < 3; ++n) {
-        const int e = lane + 64 * n;
-        if (e < kPlane) {
-            const int b = b0 - 1 + e / 9, c = c0 - 1 + e % 9;
-            if (b <= NX - 1 && c <= NY - 1) {
-                ent_kind[n] = 1;
-                ent_off[n] = (long long)node_j(b) * NY + node_k(c);
+    // Staging entries of this thread: e = tid + 64 W n.  e < 81 W: node (eb, ec) of the
+    // block of wave e / 81; then the 81 cells.  Every entry reads a valid address (nodes
+    // outside the grid read the slowness base and are replaced by Big).
+    const double *ent_base[NE];
+    bool ent_node[NE], ent_big[NE], ent_use[NE];
+#pragma unroll
+    for (int n = 0; n < NE; ++n) {
+        const int e = tid + kThreads * n;
+        ent_use[n] = e < kEnt;
+        ent_node[n] = false;
+        ent_big[n] = false;
+        ent_base[n] = A.slow;
+        if (e < kPlane * W) {
+            const int w = e / kPlane, r = e - w * kPlane;
+            const int b = b0 - 1 + r / 9, c = c0 - 1 + r % 9;
+            if (g0 + w < nact && b <= NX - 1 && c <= NY - 1) {
+                ent_node[n] = true;
+                ent_base[n] = A.tt + (long long)A.list[g0 + w] * A.node_stride +
+                              (long long)node_j(b) * NY + node_k(c);
             } else {
-                ent_kind[n] = 2;
-                ent_off[n] = 0;
+                ent_big[n] = true;
             }
-        } else if (e < kSlot) {
-            const int e2 = e - kPlane;
-            const int cb = min(b0 - 1 + e2 / 9, A.nx - 1), ccl = min(c0 - 1 + e2 % 9, A.ny - 1);
-            ent_kind[n] = 3;
-            ent_off[n] = (long long)cell_j(cb) * A.ny + cell_k(ccl);
-        } else {
-            ent_kind[n] = 0;
-            ent_off[n] = 0;
+        } else if (e < kEnt) {
+            const int r = e - kPlane * W;
+            const int cb = min(b0 - 1 + r / 9, A.nx - 1), ccl = min(c0 - 1 + r % 9, A.ny - 1);
+            ent_base[n] = A.slow + (long long)cell_j(cb) * A.ny + cell_k(ccl);
         }
     }
 
-    __shared__ double ring[kRing][kSlot];
+    __shared__ double ring[kRing][kEnt];
 
-    // plane p of the brick (p = -1 is the upwind plane a0-1) goes to slot (p+1) % kRing
-    auto load_plane = [&](int p, double v[3]) {
+    // plane p of the brick (p = -1 is the upwind plane a0-1) lives in slot (p+1) % kRing
+    auto load_plane = [&](int p, double *v) {
         const int a = a0 + p;
-        const long long tbase = (long long)node_i(a) * plane_nodes;
-        const long long sbase = (long long)cell_i(min(a, A.nz - 1)) * plane_cells;
+        const long long tb = (long long)node_i(a) * plane_nodes;
+        const long long sb = (long long)cell_i(min(a, A.nz - 1)) * plane_cells;
 #pragma unroll
-        for (int n = 0; n < 3; ++n) {
-            if (ent_kind[n] == 1) v[n] = tt[tbase + ent_off[n]];
-            else if (ent_kind[n] == 2) v[n] = kBig;
-            else if (ent_kind[n] == 3) v[n] = A.slow[sbase + ent_off[n]];
+        for (int n = 0; n < NE; ++n) {
+            const double x = ent_base[n][ent_node[n] ? tb : sb];
+            v[n] = ent_big[n] ? kBig : x;
         }
     };
-    auto store_plane = [&](int p, const double v[3]) {
+    auto store_plane = [&](int p, const double *v) {
         double *slot = ring[(p + 1) % kRing];
 #pragma unroll
-        for (int n = 0; n < 3; ++n)
-            if (ent_kind[n]) slot[lane + 64 * n] = v[n];
+        for (int n = 0; n < NE; ++n)
+            if (ent_use[n]) slot[tid + kThreads * n] = v[n];
     };
 
     for (int p = -1; p < min(kPrefetch, na); ++p) {
-        double v[3];
+        double v[NE];
         load_plane(p, v);
         store_plane(p, v);
     }
+    // planes kPrefetch .. kPrefetch+kDepth-1 in flight in registers
+    double rv[kDepth][NE];
+#pragma unroll
+    for (int k = 0; k < kDepth; ++k)
+        if (kPrefetch + k < na) load_plane(kPrefetch + k, rv[k]);
     __syncthreads();
 
+    const int tblk = wave * kPlane, sblk = W * kPlane;
     const int self = (bb + 1) * 9 + (cc + 1);  // this lane's node in a block
     const int up_b = bb * 9 + (cc + 1);        // b-1
     const int up_c = (bb + 1) * 9 + cc;        // c-1
     const int up_bc = bb * 9 + cc;             // b-1, c-1
-    const int j_self = node_j(b0 + bb), k_self = node_k(c0 + cc);
-    const long long col_off = (long long)j_self * NY + k_self;
-    const int sa = A.sz, sxv = A.sx, syv = A.sy;
-    const uint8_t code_z = sgn_code(sa), code_x = (uint8_t)(sgn_code(sxv) << 2),
-                  code_y = (uint8_t)(sgn_code(syv) << 4);
+    const long long col_off = (long long)node_j(b0 + bb) * NY + node_k(c0 + cc);
+    const uint8_t code_z = sgn_code(A.sz), code_x = (uint8_t)(sgn_code(A.sx) << 2),
+                  code_y = (uint8_t)(sgn_code(A.sy) << 4);
 
     unsigned hist = 0;  // bit k: this lane changed its node k steps ago
     bool any_change = false;
     const int nsteps = na + kSkew;
-    for (int s = 0; s < nsteps; ++s) {
-        const int pl = s + kPrefetch;
-        double v[3];
-        const bool do_load = pl < na;
-        if (do_load) load_plane(pl, v);
-
-        const int p = s - bb - cc;
-        bool changed_now = false;
-        if (inb && p >= 0 && p < na) {
-            double *cur = ring[(p + 1) % kRing];
-            const double *prv = ring[p % kRing];
-            const double t0 = cur[self];
-            int which;
-            const double tnew = node_update(
-                A, t0, prv[self], cur[up_b], cur[up_c], prv[up_b], cur[up_bc], prv[up_c],
-                prv[up_bc], prv[kPlane + up_bc], prv[kPlane + up_b], prv[kPlane + up_c],
-                prv[kPlane + self], cur[kPlane + up_bc], cur[kPlane + up_b],
-                cur[kPlane + up_c], &which);
-            cur[self] = tnew;
-            if (tnew != t0) {
-                changed_now = true;
-                if (GRAD) {
-                    uint8_t code = 0;
-                    if (which & 1) code |= code_z;
-                    if (which & 2) code |= code_x;
-                    if (which & 4) code |= code_y;
-                    sg[(long long)node_i(a0 + p) * plane_nodes + col_off] = code;
+    for (int s0 = 0; s0 < nsteps; s0 += kDepth) {
+#pragma unroll
+        for (int k = 0; k < kDepth; ++k) {
+            const int s = s0 + k;
+            const int p = s - bb - cc;
+            bool changed_now = false;
+            if (inb && p >= 0 && p < na) {
+                double *cur = ring[(p + 1) % kRing];
+                const double *prv = ring[p % kRing];
+                const double t0 = cur[tblk + self];
+                int which;
+                const double tnew = node_update(
+                    A, t0, prv[tblk + self], cur[tblk + up_b], cur[tblk + up_c],
+                    prv[tblk + up_b], cur[tblk + up_bc], prv[tblk + up_c], prv[tblk + up_bc],
+                    prv[sblk + up_bc], prv[sblk + up_b], prv[sblk + up_c], prv[sblk + self],
+                    cur[sblk + up_bc], cur[sblk + up_b], cur[sblk + up_c], &which);
+                cur[tblk + self] = tnew;
+                if (tnew != t0) {
+                    changed_now = true;
+                    if (GRAD) {
+                        uint8_t code = 0;
+                        if (which & 1) code |= code_z;
+                        if (which & 2) code |= code_x;
+                        if (which & 4) code |= code_y;
+                        sg[(long long)node_i(a0 + p) * plane_nodes + col_off] = code;
+                    }
                 }
             }
+            hist = (hist << 1) | (changed_now ? 1u : 0u);
+            any_change |= changed_now;
+
+            // write back the plane every lane has finished (only the nodes that changed)
+            const int pw = s - kSkew;
+            if (pw >= 0 && pw < na && inb && ((hist >> (kSkew - bb - cc)) & 1u))
+                tt[(long long)node_i(a0 + pw) * plane_nodes + col_off] =
+                    ring[(pw + 1) % kRing][tblk + self];
+
+            // plane s+kPrefetch leaves the registers for its slot, whose previous plane was
+            // last read one step ago; its register set takes the plane kDepth further on
+            const int pl = s + kPrefetch;
+            if (pl < na) store_plane(pl, rv[k]);
+            if (pl + kDepth < na) load_plane(pl + kDepth, rv[k]);
+            __syncthreads();
         }
-        hist = (hist << 1) | (changed_now ? 1u : 0u);
-        any_change |= changed_now;
-
-        // write back the plane every lane has finished
-        const int pw = s - kSkew;
-        if (pw >= 0 && inb && ((hist >> (kSkew - bb - cc)) & 1u))
-            tt[(long long)node_i(a0 + pw) * plane_nodes + col_off] =
-                ring[(pw + 1) % kRing][self];
-
-        if (do_load) store_plane(pl, v);
-        __syncthreads();
     }
 
-    if (__any(any_change) && lane == 0) A.changed[src] = 1;
+    if (__any(any_change) && lane == 0 && has_src) A.changed[src] = 1;
 }
 
 // tt = Big everywhere (and signs cleared), _fteik3d.py:429-437
@@ -310,7 +332,7 @@ __device__ __forceinline__ double t_ana3(int i, int j, int k, double dz, double 
 __global__ void seed3d_kernel(const double *slow, int nz, int nx, int ny, double dz, double dx,
                               double dy, const SrcParam *prm, int nsrc, double *tt,
                               long long node_stride, double *vzero, int *sets_done,
-                              int *changed, int *active) {
+                              int *changed, int *active, int *list, int *nlist) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nsrc * 8) return;
     const int s = g >> 3, c = g & 7;
@@ -328,28 +350,30 @@ __global__ void seed3d_kernel(const double *slow, int nz, int nx, int ny, double
         sets_done[s] = 0;
         changed[s] = 0;
         active[s] = 1;
+        list[s] = s;
+        if (s == 0) *nlist = nsrc;
     }
 }
 
 // After each set: count it for every active source, retire sources whose set changed
-// nothing (further sets would repeat it exactly), clear the change flags.  One block.
+// nothing (further sets would repeat it exactly), clear the change flags and rebuild the
+// compacted list of sources the next set sweeps.  One block.
 __global__ void set_bookkeeping_kernel(int nsrc, int *sets_done, int *changed, int *active,
-                                       int *nactive) {
-    __shared__ int cnt;
-    if (threadIdx.x == 0) cnt = 0;
-    __syncthreads();
-    int mine = 0;
+                                       int *list, int *nlist) {
     for (int s = threadIdx.x; s < nsrc; s += blockDim.x) {
         if (active[s]) {
             sets_done[s] += 1;
             if (!changed[s]) active[s] = 0;
-            else ++mine;
         }
         changed[s] = 0;
     }
-    atomicAdd(&cnt, mine);
     __syncthreads();
-    if (threadIdx.x == 0) *nactive = cnt;
+    if (threadIdx.x == 0) {
+        int n = 0;
+        for (int s = 0; s < nsrc; ++s)
+            if (active[s]) list[n++] = s;
+        *nlist = n;
+    }
 }
 
 // Gradient post-pass, _fteik3d.py:464-488: components with a recorded upwind sign are
@@ -427,9 +451,27 @@ int bricks_on_diag(int d, int nba, int nbb, int nbc) {
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
+template <bool G, int W>
+void launch_brick_t(dim3 grid, hipStream_t st, const SweepArgs &a) {
+    hipLaunchKernelGGL((sweep3d_brick_kernel<G, W>), grid, dim3(64 * W), 0, st, a);
+}
+
+void launch_brick(bool grad, int group, dim3 grid, hipStream_t st, const SweepArgs &a) {
+    switch (group * 2 + (grad ? 1 : 0)) {
+        case 2: launch_brick_t<false, 1>(grid, st, a); break;
+        case 3: launch_brick_t<true, 1>(grid, st, a); break;
+        case 4: launch_brick_t<false, 2>(grid, st, a); break;
+        case 5: launch_brick_t<true, 2>(grid, st, a); break;
+        case 8: launch_brick_t<false, 4>(grid, st, a); break;
+        case 9: launch_brick_t<true, 4>(grid, st, a); break;
+        case 16: launch_brick_t<false, 8>(grid, st, a); break;
+        default: launch_brick_t<true, 8>(grid, st, a); break;
+    }
+}
+
 struct Ws3 {
     SrcParam *prm;
-    int *changed, *active, *nactive;
+    int *changed, *active, *list, *nlist;
 };
 
 Ws3 carve3(void *ws, int nsrc) {
@@ -441,7 +483,9 @@ Ws3 carve3(void *ws, int nsrc) {
     p += align256(sizeof(int) * nsrc);
     w.active = (int *)p;
     p += align256(sizeof(int) * nsrc);
-    w.nactive = (int *)p;
+    w.list = (int *)p;
+    p += align256(sizeof(int) * nsrc);
+    w.nlist = (int *)p;
     return w;
 }
 
@@ -453,12 +497,12 @@ using namespace fteik;
 extern "C" const char *fteik_version(void) { return "fteik_amd 0.1.0 (gfx950)"; }
 
 extern "C" size_t fteik3d_workspace_size(int nsrc) {
-    return align256(sizeof(SrcParam) * nsrc) + 2 * align256(sizeof(int) * nsrc) + 256;
+    return align256(sizeof(SrcParam) * nsrc) + 3 * align256(sizeof(int) * nsrc) + 256;
 }
 
 extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double dz, double dx,
                              double dy, const double *src, int nsrc, int nsweep, int max_sets,
-                             int want_grad, int tile_a, double *tt, uint8_t *sgn,
+                             int want_grad, int tile_a, int group, double *tt, uint8_t *sgn,
                              double *vzero, int *sets_done, void *workspace,
                              size_t workspace_bytes, int use_graph, void *stream_) {
     if (nz < 1 || nx < 1 || ny < 1 || nsrc < 1 || nsrc > 65535 || !slow || !src || !tt ||
@@ -468,6 +512,8 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
     if (!(dz > 0.0) || !(dx > 0.0) || !(dy > 0.0)) return FTEIK_ERR_ARG;
     if (workspace_bytes < fteik3d_workspace_size(nsrc)) return FTEIK_ERR_WORKSPACE;
     if (nsweep <= 0 && max_sets < 1) return FTEIK_ERR_ARG;
+    if (group == 0) group = nsrc >= 64 ? 4 : (nsrc >= 8 ? 2 : 1);
+    if (group != 1 && group != 2 && group != 4 && group != 8) return FTEIK_ERR_ARG;
     hipStream_t stream = (hipStream_t)stream_;
 
     std::vector<SrcParam> prm(nsrc);
@@ -493,7 +539,7 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
         FTEIK_CHECK(hipGetLastError());
         hipLaunchKernelGGL(seed3d_kernel, dim3((nsrc * 8 + 255) / 256), dim3(256), 0, stream,
                            slow, nz, nx, ny, dz, dx, dy, w.prm, nsrc, tt, nn, vzero, sets_done,
-                           w.changed, w.active);
+                           w.changed, w.active, w.list, w.nlist);
         FTEIK_CHECK(hipGetLastError());
     }
 
@@ -502,7 +548,8 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
     a.tt = tt;
     a.sgn = want_grad ? sgn : nullptr;
     a.changed = w.changed;
-    a.active = w.active;
+    a.list = w.list;
+    a.nlist = w.nlist;
     a.node_stride = nn;
     a.nz = nz; a.nx = nx; a.ny = ny;
     a.NZ = NZ; a.NX = NX; a.NY = NY;
@@ -533,16 +580,11 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
             a.sy = dirs[p][2];
             for (int d = 0; d < ndiag; ++d) {
                 a.diag = d;
-                if (want_grad)
-                    hipLaunchKernelGGL(sweep3d_brick_kernel<true>, dim3(nb[d], nsrc), dim3(64), 0,
-                                       stream, a);
-                else
-                    hipLaunchKernelGGL(sweep3d_brick_kernel<false>, dim3(nb[d], nsrc), dim3(64),
-                                       0, stream, a);
+                launch_brick(want_grad != 0, group, dim3(nb[d], ceil_div(nsrc, group)), stream, a);
             }
         }
         hipLaunchKernelGGL(set_bookkeeping_kernel, dim3(1), dim3(256), 0, stream, nsrc, sets_done,
-                           w.changed, w.active, w.nactive);
+                           w.changed, w.active, w.list, w.nlist);
         return hip_status(hipGetLastError());
     };
 
@@ -570,7 +612,7 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
         for (int s = 0; s < max_sets && nact > 0 && !status; ++s) {
             status = run_set();
             if (status) break;
-            hipError_t e = hipMemcpyAsync(&nact, w.nactive, sizeof(int), hipMemcpyDeviceToHost, stream);
+            hipError_t e = hipMemcpyAsync(&nact, w.nlist, sizeof(int), hipMemcpyDeviceToHost, stream);
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
             status = hip_status(e);
         }

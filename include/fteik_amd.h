@@ -51,6 +51,7 @@ size_t fteik3d_workspace_size(int nsrc);
  *           <= 0: sets until one changes no node, at most max_sets
  * want_grad record the upwind direction of every update (needed by fteik3d_gradient)
  * tile_a    nodes per brick along z (0 = library default for the shape)
+ * group     sources per workgroup sharing the staged slowness (0 = default; 1, 2, 4, 8)
  * tt        device [nsrc, nz+1, nx+1, ny+1] traveltimes (output)
  * sgn       device [nsrc, nz+1, nx+1, ny+1] uint8 packed upwind signs or NULL
  * vzero     device [nsrc] slowness of each source cell (output)
@@ -63,7 +64,7 @@ size_t fteik3d_workspace_size(int nsrc);
  */
 int fteik3d_solve(const double *slow, int nz, int nx, int ny, double dz, double dx, double dy,
                   const double *src, int nsrc, int nsweep, int max_sets, int want_grad,
-                  int tile_a, double *tt, uint8_t *sgn, double *vzero, int *sets_done,
+                  int tile_a, int group, double *tt, uint8_t *sgn, double *vzero, int *sets_done,
                   void *workspace, size_t workspace_bytes, int use_graph, void *stream);
 
 /*

@@ -96,14 +96,17 @@ def _rough_model(shape, seed):
     return 1500.0 * np.exp(0.8 * p / np.abs(p).max())
 
 
-@pytest.mark.parametrize("tile_a", [1, 3, 8, 0, 64])
-def test_brick_shapes_bitwise(fa, tile_a):
-    """Partial bricks in every axis, several z brick heights, 3 sources in one launch."""
+@pytest.mark.parametrize("tile_a, group", [(1, 1), (3, 2), (8, 4), (0, 0), (64, 8), (5, 4)])
+def test_brick_shapes_bitwise(fa, tile_a, group):
+    """Partial bricks in every axis, several z brick heights and source groups (groups
+    partly empty), sources converging after different numbers of sets."""
     shape, d = (21, 19, 27), (7.0, 9.0, 8.0)
     v = _rough_model(shape, 11)
-    srcs = np.array([[3.3, 17.7, 40.1], [140.0, 60.0, 100.0], [147.0, 171.0, 216.0]])
+    srcs = np.array([[3.3, 17.7, 40.1], [140.0, 60.0, 100.0], [147.0, 171.0, 216.0],
+                     [70.0, 85.5, 0.0], [0.0, 0.0, 0.0]])
     ref, _, _, kref = oracle.solve3d(1.0 / v, *d, srcs, nsweep=0)
-    tt, _, _, sets = fa.Eikonal3D(v, d).solve_tensors(srcs, nsweep=None, tile_a=tile_a)
+    tt, _, _, sets = fa.Eikonal3D(v, d).solve_tensors(srcs, nsweep=None, tile_a=tile_a,
+                                                      group=group)
     assert list(sets) == list(kref)
     assert np.array_equal(tt.cpu().numpy(), ref)
 
