@@ -31,7 +31,7 @@ ERR_WORKSPACE = 5
 # name -> (restype, argtypes); every entry point of include/fteik_amd.h
 SIGNATURES = {
     "fteik_version": (ctypes.c_char_p, []),
-    "fteik3d_workspace_size": (ctypes.c_size_t, [_c_int]),
+    "fteik3d_workspace_size": (ctypes.c_size_t, [_c_int, _c_int, _c_int, _c_int]),
     "fteik3d_solve": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _dp, _c_int,
                                _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp,
                                ctypes.c_size_t, _c_int, _vp]),

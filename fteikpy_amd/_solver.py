@@ -110,7 +110,7 @@ class Eikonal3D(BaseGrid3D, _SlownessCache):
         vzero = torch.empty(n, dtype=torch.float64, device=dev)
         sets = torch.empty(n, dtype=torch.int32, device=dev)
         L = _lib.lib()
-        wsb = L.fteik3d_workspace_size(n)
+        wsb = L.fteik3d_workspace_size(nz, nx, ny, n)
         ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
         slow = self._slowness()
         args = (_ptr(slow), nz, nx, ny, *self._gridsize, rp, n, _nsweep(nsweep), int(max_sets),

@@ -37,9 +37,12 @@ struct SweepArgs {
     const double *slow;
     double *tt;
     uint8_t *sgn;
+    uint8_t *chg;           // per node: bit p set if the node changed in the last pass p
     int *changed;
     const int *list;        // sources still changing (compacted), first *nlist entries
     const int *nlist;
+    const int *sets_done;   // sets completed per source (0 during the first set)
+    int pass;               // 0..7, position of this pass in the set
     long long node_stride;  // nodes per source grid
     int nz, nx, ny;         // cells
     int NZ, NX, NY;         // nodes
@@ -193,21 +196,26 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
     // block of wave e / 81; then the 81 cells.  Every entry reads a valid address (nodes
     // outside the grid read the slowness base and are replaced by Big).
     const double *ent_base[NE];
-    bool ent_node[NE], ent_big[NE], ent_use[NE];
+    const uint8_t *ent_chg[NE];
+    bool ent_node[NE], ent_big[NE], ent_use[NE], ent_c[NE];
 #pragma unroll
     for (int n = 0; n < NE; ++n) {
         const int e = tid + kThreads * n;
         ent_use[n] = e < kEnt;
+        ent_c[n] = e < kPlane * W;
         ent_node[n] = false;
         ent_big[n] = false;
         ent_base[n] = A.slow;
+        ent_chg[n] = A.chg;
         if (e < kPlane * W) {
             const int w = e / kPlane, r = e - w * kPlane;
             const int b = b0 - 1 + r / 9, c = c0 - 1 + r % 9;
             if (g0 + w < nact && b <= NX - 1 && c <= NY - 1) {
                 ent_node[n] = true;
-                ent_base[n] = A.tt + (long long)A.list[g0 + w] * A.node_stride +
-                              (long long)node_j(b) * NY + node_k(c);
+                const long long off = (long long)A.list[g0 + w] * A.node_stride +
+                                      (long long)node_j(b) * NY + node_k(c);
+                ent_base[n] = A.tt + off;
+                ent_chg[n] = A.chg + off;
             } else {
                 ent_big[n] = true;
             }
@@ -219,9 +227,10 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
     }
 
     __shared__ double ring[kRing][kEnt];
+    __shared__ uint8_t ringc[kRing][kPlane * W];
 
     // plane p of the brick (p = -1 is the upwind plane a0-1) lives in slot (p+1) % kRing
-    auto load_plane = [&](int p, double *v) {
+    auto load_plane = [&](int p, double *v, unsigned *cv) {
         const int a = a0 + p;
         const long long tb = (long long)node_i(a) * plane_nodes;
         const long long sb = (long long)cell_i(min(a, A.nz - 1)) * plane_cells;
@@ -229,26 +238,39 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
         for (int n = 0; n < NE; ++n) {
             const double x = ent_base[n][ent_node[n] ? tb : sb];
             v[n] = ent_big[n] ? kBig : x;
+            const unsigned c = ent_chg[n][ent_node[n] ? tb : 0];
+            cv[n] = ent_node[n] ? c : 0u;
         }
     };
-    auto store_plane = [&](int p, const double *v) {
+    auto store_plane = [&](int p, const double *v, const unsigned *cv) {
         double *slot = ring[(p + 1) % kRing];
+        uint8_t *cslot = ringc[(p + 1) % kRing];
 #pragma unroll
-        for (int n = 0; n < NE; ++n)
+        for (int n = 0; n < NE; ++n) {
             if (ent_use[n]) slot[tid + kThreads * n] = v[n];
+            if (ent_c[n]) cslot[tid + kThreads * n] = (uint8_t)cv[n];
+        }
     };
 
     for (int p = -1; p < min(kPrefetch, na); ++p) {
         double v[NE];
-        load_plane(p, v);
-        store_plane(p, v);
+        unsigned cv[NE];
+        load_plane(p, v, cv);
+        store_plane(p, v, cv);
     }
     // planes kPrefetch .. kPrefetch+kDepth-1 in flight in registers
     double rv[kDepth][NE];
+    unsigned rc[kDepth][NE];
 #pragma unroll
     for (int k = 0; k < kDepth; ++k)
-        if (kPrefetch + k < na) load_plane(kPrefetch + k, rv[k]);
+        if (kPrefetch + k < na) load_plane(kPrefetch + k, rv[k], rc[k]);
     __syncthreads();
+    // From the second set on, a node none of whose 7 upwind neighbours changed during the
+    // last 8 passes sees exactly the candidates of its previous update in this direction,
+    // while its own time can only have decreased since: it cannot change and is skipped.
+    const bool may_skip = has_src && A.sets_done[src] > 0;
+    const unsigned pbit = 1u << A.pass;
+    uint8_t *cg = A.chg + (long long)src * A.node_stride;
 
     const int tblk = wave * kPlane, sblk = W * kPlane;
     const int self = (bb + 1) * 9 + (cc + 1);  // this lane's node in a block
@@ -259,7 +281,8 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
     const uint8_t code_z = sgn_code(A.sz), code_x = (uint8_t)(sgn_code(A.sx) << 2),
                   code_y = (uint8_t)(sgn_code(A.sy) << 4);
 
-    unsigned hist = 0;  // bit k: this lane changed its node k steps ago
+    unsigned hist = 0;   // bit k: this lane changed its node k steps ago
+    unsigned chist = 0;  // bit k: this lane changed its node's change bits k steps ago
     bool any_change = false;
     const int nsteps = na + kSkew;
     for (int s0 = 0; s0 < nsteps; s0 += kDepth) {
@@ -267,11 +290,16 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
         for (int k = 0; k < kDepth; ++k) {
             const int s = s0 + k;
             const int p = s - bb - cc;
-            bool changed_now = false;
+            bool changed_now = false, cchanged = false;
             if (inb && p >= 0 && p < na) {
                 double *cur = ring[(p + 1) % kRing];
                 const double *prv = ring[p % kRing];
+                uint8_t *curc = ringc[(p + 1) % kRing] + tblk;
+                const uint8_t *prvc = ringc[p % kRing] + tblk;
+                const unsigned up = curc[up_b] | curc[up_c] | curc[up_bc] | prvc[self] |
+                                    prvc[up_b] | prvc[up_c] | prvc[up_bc];
                 const double t0 = cur[tblk + self];
+                if (!may_skip || up != 0u) {
                 int which;
                 const double tnew = node_update(
                     A, t0, prv[tblk + self], cur[tblk + up_b], cur[tblk + up_c],
@@ -289,21 +317,31 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
                         sg[(long long)node_i(a0 + p) * plane_nodes + col_off] = code;
                     }
                 }
+                }
+                const unsigned cs = curc[self];
+                const unsigned cn = (cs & ~pbit) | (changed_now ? pbit : 0u);
+                if (cn != cs) {
+                    curc[self] = (uint8_t)cn;
+                    cchanged = true;
+                }
             }
             hist = (hist << 1) | (changed_now ? 1u : 0u);
+            chist = (chist << 1) | (cchanged ? 1u : 0u);
             any_change |= changed_now;
 
             // write back the plane every lane has finished (only the nodes that changed)
             const int pw = s - kSkew;
-            if (pw >= 0 && pw < na && inb && ((hist >> (kSkew - bb - cc)) & 1u))
-                tt[(long long)node_i(a0 + pw) * plane_nodes + col_off] =
-                    ring[(pw + 1) % kRing][tblk + self];
+            if (pw >= 0 && pw < na && inb) {
+                const long long g = (long long)node_i(a0 + pw) * plane_nodes + col_off;
+                if ((hist >> (kSkew - bb - cc)) & 1u) tt[g] = ring[(pw + 1) % kRing][tblk + self];
+                if ((chist >> (kSkew - bb - cc)) & 1u) cg[g] = ringc[(pw + 1) % kRing][tblk + self];
+            }
 
             // plane s+kPrefetch leaves the registers for its slot, whose previous plane was
             // last read one step ago; its register set takes the plane kDepth further on
             const int pl = s + kPrefetch;
-            if (pl < na) store_plane(pl, rv[k]);
-            if (pl + kDepth < na) load_plane(pl + kDepth, rv[k]);
+            if (pl < na) store_plane(pl, rv[k], rc[k]);
+            if (pl + kDepth < na) load_plane(pl + kDepth, rv[k], rc[k]);
             __syncthreads();
         }
     }
@@ -312,10 +350,11 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
 }
 
 // tt = Big everywhere (and signs cleared), _fteik3d.py:429-437
-__global__ void fill_kernel(double *tt, long long n, uint8_t *sgn) {
+__global__ void fill_kernel(double *tt, long long n, uint8_t *sgn, uint8_t *chg) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         tt[i] = kBig;
+        chg[i] = 0;
         if (sgn) sgn[i] = 0;
     }
 }
@@ -472,9 +511,10 @@ void launch_brick(bool grad, int group, dim3 grid, hipStream_t st, const SweepAr
 struct Ws3 {
     SrcParam *prm;
     int *changed, *active, *list, *nlist;
+    uint8_t *chg;
 };
 
-Ws3 carve3(void *ws, int nsrc) {
+Ws3 carve3(void *ws, int nsrc, long long nodes) {
     char *p = (char *)ws;
     Ws3 w;
     w.prm = (SrcParam *)p;
@@ -486,6 +526,9 @@ Ws3 carve3(void *ws, int nsrc) {
     w.list = (int *)p;
     p += align256(sizeof(int) * nsrc);
     w.nlist = (int *)p;
+    p += 256;
+    w.chg = (uint8_t *)p;
+    (void)nodes;
     return w;
 }
 
@@ -496,8 +539,10 @@ using namespace fteik;
 
 extern "C" const char *fteik_version(void) { return "fteik_amd 0.1.0 (gfx950)"; }
 
-extern "C" size_t fteik3d_workspace_size(int nsrc) {
-    return align256(sizeof(SrcParam) * nsrc) + 3 * align256(sizeof(int) * nsrc) + 256;
+extern "C" size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc) {
+    const size_t nodes = (size_t)(nz + 1) * (nx + 1) * (ny + 1);
+    return align256(sizeof(SrcParam) * nsrc) + 3 * align256(sizeof(int) * nsrc) + 256 +
+           align256(nodes * nsrc);
 }
 
 extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double dz, double dx,
@@ -510,7 +555,7 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
         return FTEIK_ERR_ARG;
     if (want_grad && !sgn) return FTEIK_ERR_ARG;
     if (!(dz > 0.0) || !(dx > 0.0) || !(dy > 0.0)) return FTEIK_ERR_ARG;
-    if (workspace_bytes < fteik3d_workspace_size(nsrc)) return FTEIK_ERR_WORKSPACE;
+    if (workspace_bytes < fteik3d_workspace_size(nz, nx, ny, nsrc)) return FTEIK_ERR_WORKSPACE;
     if (nsweep <= 0 && max_sets < 1) return FTEIK_ERR_ARG;
     if (group == 0) group = nsrc >= 64 ? 4 : (nsrc >= 8 ? 2 : 1);
     if (group != 1 && group != 2 && group != 4 && group != 8) return FTEIK_ERR_ARG;
@@ -523,7 +568,7 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
 
     const int NZ = nz + 1, NX = nx + 1, NY = ny + 1;
     const long long nn = (long long)NZ * NX * NY;
-    Ws3 w = carve3(workspace, nsrc);
+    Ws3 w = carve3(workspace, nsrc, nn);
     FTEIK_CHECK(hipMemcpyAsync(w.prm, prm.data(), sizeof(SrcParam) * nsrc, hipMemcpyHostToDevice,
                                stream));
     // the host vector goes out of scope on return: finish the copy first
@@ -535,7 +580,7 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
         const long long want = (total + threads - 1) / threads;
         const int blocks = (int)std::min<long long>(want, 65536);
         hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(threads), 0, stream, tt, total,
-                           want_grad ? sgn : (uint8_t *)nullptr);
+                           want_grad ? sgn : (uint8_t *)nullptr, w.chg);
         FTEIK_CHECK(hipGetLastError());
         hipLaunchKernelGGL(seed3d_kernel, dim3((nsrc * 8 + 255) / 256), dim3(256), 0, stream,
                            slow, nz, nx, ny, dz, dx, dy, w.prm, nsrc, tt, nn, vzero, sets_done,
@@ -550,6 +595,8 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
     a.changed = w.changed;
     a.list = w.list;
     a.nlist = w.nlist;
+    a.chg = w.chg;
+    a.sets_done = sets_done;
     a.node_stride = nn;
     a.nz = nz; a.nx = nx; a.ny = ny;
     a.NZ = NZ; a.NX = NX; a.NY = NY;
@@ -578,6 +625,7 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
             a.sz = dirs[p][0];
             a.sx = dirs[p][1];
             a.sy = dirs[p][2];
+            a.pass = p;
             for (int d = 0; d < ndiag; ++d) {
                 a.diag = d;
                 launch_brick(want_grad != 0, group, dim3(nb[d], ceil_div(nsrc, group)), stream, a);

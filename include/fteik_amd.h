@@ -38,8 +38,9 @@ const char *fteik_version(void);
 /* 3D solve                                                                            */
 /* ---------------------------------------------------------------------------------- */
 
-/* Device workspace (bytes) fteik3d_solve needs for nsrc sources. */
-size_t fteik3d_workspace_size(int nsrc);
+/* Device workspace (bytes) fteik3d_solve needs for nsrc sources on an nz x nx x ny
+ * (cells) grid: per-source bookkeeping plus one byte per node and source. */
+size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc);
 
 /*
  * Replaces solve3d / fteik3d / fteik3d_vectorized (fteikpy/_fteik/_fteik3d.py:394-525),

@@ -33,8 +33,8 @@ def test_workspace_sizes():
     from fteikpy_amd import _lib
 
     L = _lib.lib()
-    assert L.fteik3d_workspace_size(1) > 0
-    assert L.fteik3d_workspace_size(256) > L.fteik3d_workspace_size(1)
+    assert L.fteik3d_workspace_size(4, 4, 4, 1) >= 125
+    assert L.fteik3d_workspace_size(4, 4, 4, 256) > L.fteik3d_workspace_size(4, 4, 4, 1)
     assert L.fteik2d_workspace_size(4) > 0
 
 
@@ -67,7 +67,7 @@ def test_source_check_before_any_device_work():
     fake = ctypes.c_void_p(0x1000)
     src = np.array([-1.0, 0.0, 0.0])
     sp = src.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-    ws = L.fteik3d_workspace_size(1)
+    ws = L.fteik3d_workspace_size(4, 4, 4, 1)
     st = L.fteik3d_solve(fake, 4, 4, 4, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, 0, fake, None, fake, fake,
                          fake, ws, 0, None)
     assert st == _lib.ERR_SOURCE_OOB
