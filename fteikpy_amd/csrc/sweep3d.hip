@@ -38,6 +38,7 @@ struct SweepArgs {
     double *tt;
     uint8_t *sgn;
     uint8_t *chg;           // per node: bit p set if the node changed in the last pass p
+    uint8_t *tsum;          // per source and tile: OR of the tile nodes' change bits
     int *changed;
     const int *list;        // sources still changing (compacted), first *nlist entries
     const int *nlist;
@@ -47,8 +48,8 @@ struct SweepArgs {
     int nz, nx, ny;         // cells
     int NZ, NX, NY;         // nodes
     int sz, sx, sy;         // pass direction (sgnt of the reference)
-    int tile_a;             // planes per brick
-    int nba, nbb, nbc;      // brick counts
+    int tile_a;             // z extent of a tile / brick
+    int nba, nbb, nbc;      // tile counts along z, x, y (memory grid, nodes 0..N-1)
     int diag;               // brick anti-diagonal of this launch
     double dz, dx, dy, dz2i, dx2i, dy2i, dz2dx2, dz2dy2, dx2dy2, dsum;
 };
@@ -140,6 +141,12 @@ __device__ __forceinline__ double node_update(const SweepArgs &A, double t0, dou
 // Each workgroup runs W waves on the same brick for W different sources (from the
 // compacted list of sources still changing): the slowness block of every staged plane is
 // loaded once into LDS and shared, each wave owns the traveltime block of its source.
+//
+// Bricks are the tiles of one fixed grid in memory coordinates (tile_a x 8 x 8 nodes), so
+// every pass visits the same tiles whatever its direction; a per-source byte per tile
+// records in which of the last 8 passes any of its nodes changed.  A tile whose own byte
+// and the bytes of its upwind neighbour tiles are all zero contains only nodes that the
+// node-level rule below would skip, so the whole brick is skipped.
 template <bool GRAD, int W>
 __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A) {
     constexpr int kEnt = kPlane * (W + 1);  // staged entries per plane: W node blocks + cells
@@ -169,23 +176,59 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
         }
     }
 
+    const int NX = A.NX, NY = A.NY;
+    // oriented brick origin: tile BA/BB/BC of the memory grid seen from this direction
+    const int a0 = A.tile_a * BA - (A.sz > 0 ? 0 : A.tile_a * A.nba - A.NZ);
+    const int b0 = kLanesB * BB - (A.sx > 0 ? 0 : kLanesB * A.nbb - NX);
+    const int c0 = kLanesC * BC - (A.sy > 0 ? 0 : kLanesC * A.nbc - NY);
+    const int a_first = max(1, a0);
+    const int na = min(A.NZ - 1, a0 + A.tile_a - 1) - a_first + 1;  // planes to update
+    if (na <= 0 || b0 + kLanesB - 1 < 1 || c0 + kLanesC - 1 < 1) return;
+    // tile of the memory grid and its upwind neighbours
+    const int ti = A.sz > 0 ? BA : A.nba - 1 - BA;
+    const int tj = A.sx > 0 ? BB : A.nbb - 1 - BB;
+    const int tk = A.sy > 0 ? BC : A.nbc - 1 - BC;
+    const long long tiles = (long long)A.nba * A.nbb * A.nbc;
+
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const bool has_src = g0 + wave < nact;
+    __shared__ int wave_on[W];
+    {
+        const bool has = g0 + wave < nact;
+        const int s0 = has ? A.list[g0 + wave] : 0;
+        bool on = has;
+        if (has && A.sets_done[s0] > 0) {
+            // lanes 0..7 read the tile byte and its 7 upwind neighbours
+            unsigned any = 0;
+            if (lane < 8) {
+                const int ii = ti - ((lane & 1) ? A.sz : 0);
+                const int jj = tj - ((lane & 2) ? A.sx : 0);
+                const int kk = tk - ((lane & 4) ? A.sy : 0);
+                if (ii >= 0 && ii < A.nba && jj >= 0 && jj < A.nbb && kk >= 0 && kk < A.nbc)
+                    any = A.tsum[s0 * tiles + ((long long)ii * A.nbb + jj) * A.nbc + kk];
+            }
+            on = __any(any != 0u);
+        }
+        if (lane == 0) wave_on[wave] = on ? 1 : 0;
+    }
+    __syncthreads();
+    bool block_on = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) block_on |= wave_on[w] != 0;
+    if (!block_on) return;  // block-uniform
+
+    const bool has_src = wave_on[wave] != 0;
     const int src = has_src ? A.list[g0 + wave] : 0;
     const int bb = lane >> 3, cc = lane & 7;
-    const int a0 = 1 + BA * A.tile_a;        // first oriented plane of the brick
-    const int na = min(A.tile_a, A.NZ - a0);  // planes a0 .. a0+na-1
-    const int b0 = 1 + BB * kLanesB, c0 = 1 + BC * kLanesC;
-    const bool inb = has_src && (b0 + bb <= A.NX - 1) && (c0 + cc <= A.NY - 1);
+    const int b = b0 + bb, c = c0 + cc;
+    const bool inb = has_src && b >= 1 && b <= NX - 1 && c >= 1 && c <= NY - 1;
 
     // oriented -> memory index maps
-    const int NX = A.NX, NY = A.NY;
     auto node_i = [&](int a) { return A.sz > 0 ? a : A.NZ - 1 - a; };
-    auto node_j = [&](int b) { return A.sx > 0 ? b : A.NX - 1 - b; };
-    auto node_k = [&](int c) { return A.sy > 0 ? c : A.NY - 1 - c; };
+    auto node_j = [&](int x) { return A.sx > 0 ? x : NX - 1 - x; };
+    auto node_k = [&](int y) { return A.sy > 0 ? y : NY - 1 - y; };
     auto cell_i = [&](int a) { return A.sz > 0 ? a : A.nz - 1 - a; };
-    auto cell_j = [&](int b) { return A.sx > 0 ? b : A.nx - 1 - b; };
-    auto cell_k = [&](int c) { return A.sy > 0 ? c : A.ny - 1 - c; };
+    auto cell_j = [&](int x) { return A.sx > 0 ? x : A.nx - 1 - x; };
+    auto cell_k = [&](int y) { return A.sy > 0 ? y : A.ny - 1 - y; };
 
     double *tt = A.tt + (long long)src * A.node_stride;
     uint8_t *sg = GRAD ? A.sgn + (long long)src * A.node_stride : nullptr;
@@ -209,11 +252,11 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
         ent_chg[n] = A.chg;
         if (e < kPlane * W) {
             const int w = e / kPlane, r = e - w * kPlane;
-            const int b = b0 - 1 + r / 9, c = c0 - 1 + r % 9;
-            if (g0 + w < nact && b <= NX - 1 && c <= NY - 1) {
+            const int eb = b0 - 1 + r / 9, ec = c0 - 1 + r % 9;
+            if (wave_on[w] && eb >= 0 && eb <= NX - 1 && ec >= 0 && ec <= NY - 1) {
                 ent_node[n] = true;
                 const long long off = (long long)A.list[g0 + w] * A.node_stride +
-                                      (long long)node_j(b) * NY + node_k(c);
+                                      (long long)node_j(eb) * NY + node_k(ec);
                 ent_base[n] = A.tt + off;
                 ent_chg[n] = A.chg + off;
             } else {
@@ -221,7 +264,8 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
             }
         } else if (e < kEnt) {
             const int r = e - kPlane * W;
-            const int cb = min(b0 - 1 + r / 9, A.nx - 1), ccl = min(c0 - 1 + r % 9, A.ny - 1);
+            const int cb = min(max(b0 - 1 + r / 9, 0), A.nx - 1);
+            const int ccl = min(max(c0 - 1 + r % 9, 0), A.ny - 1);
             ent_base[n] = A.slow + (long long)cell_j(cb) * A.ny + cell_k(ccl);
         }
     }
@@ -229,17 +273,18 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
     __shared__ double ring[kRing][kEnt];
     __shared__ uint8_t ringc[kRing][kPlane * W];
 
-    // plane p of the brick (p = -1 is the upwind plane a0-1) lives in slot (p+1) % kRing
+    // plane p of the brick (a = a_first + p; p = -1 is the upwind plane) lives in slot
+    // (p+1) % kRing
     auto load_plane = [&](int p, double *v, unsigned *cv) {
-        const int a = a0 + p;
+        const int a = a_first + p;
         const long long tb = (long long)node_i(a) * plane_nodes;
         const long long sb = (long long)cell_i(min(a, A.nz - 1)) * plane_cells;
 #pragma unroll
         for (int n = 0; n < NE; ++n) {
             const double x = ent_base[n][ent_node[n] ? tb : sb];
             v[n] = ent_big[n] ? kBig : x;
-            const unsigned c = ent_chg[n][ent_node[n] ? tb : 0];
-            cv[n] = ent_node[n] ? c : 0u;
+            const unsigned cb = ent_chg[n][ent_node[n] ? tb : 0];
+            cv[n] = ent_node[n] ? cb : 0u;
         }
     };
     auto store_plane = [&](int p, const double *v, const unsigned *cv) {
@@ -277,7 +322,7 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
     const int up_b = bb * 9 + (cc + 1);        // b-1
     const int up_c = (bb + 1) * 9 + cc;        // c-1
     const int up_bc = bb * 9 + cc;             // b-1, c-1
-    const long long col_off = (long long)node_j(b0 + bb) * NY + node_k(c0 + cc);
+    const long long col_off = inb ? (long long)node_j(b) * NY + node_k(c) : 0;
     const uint8_t code_z = sgn_code(A.sz), code_x = (uint8_t)(sgn_code(A.sx) << 2),
                   code_y = (uint8_t)(sgn_code(A.sy) << 4);
 
@@ -300,23 +345,23 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
                                     prvc[up_b] | prvc[up_c] | prvc[up_bc];
                 const double t0 = cur[tblk + self];
                 if (!may_skip || up != 0u) {
-                int which;
-                const double tnew = node_update(
-                    A, t0, prv[tblk + self], cur[tblk + up_b], cur[tblk + up_c],
-                    prv[tblk + up_b], cur[tblk + up_bc], prv[tblk + up_c], prv[tblk + up_bc],
-                    prv[sblk + up_bc], prv[sblk + up_b], prv[sblk + up_c], prv[sblk + self],
-                    cur[sblk + up_bc], cur[sblk + up_b], cur[sblk + up_c], &which);
-                cur[tblk + self] = tnew;
-                if (tnew != t0) {
-                    changed_now = true;
-                    if (GRAD) {
-                        uint8_t code = 0;
-                        if (which & 1) code |= code_z;
-                        if (which & 2) code |= code_x;
-                        if (which & 4) code |= code_y;
-                        sg[(long long)node_i(a0 + p) * plane_nodes + col_off] = code;
+                    int which;
+                    const double tnew = node_update(
+                        A, t0, prv[tblk + self], cur[tblk + up_b], cur[tblk + up_c],
+                        prv[tblk + up_b], cur[tblk + up_bc], prv[tblk + up_c], prv[tblk + up_bc],
+                        prv[sblk + up_bc], prv[sblk + up_b], prv[sblk + up_c], prv[sblk + self],
+                        cur[sblk + up_bc], cur[sblk + up_b], cur[sblk + up_c], &which);
+                    cur[tblk + self] = tnew;
+                    if (tnew != t0) {
+                        changed_now = true;
+                        if (GRAD) {
+                            uint8_t code = 0;
+                            if (which & 1) code |= code_z;
+                            if (which & 2) code |= code_x;
+                            if (which & 4) code |= code_y;
+                            sg[(long long)node_i(a_first + p) * plane_nodes + col_off] = code;
+                        }
                     }
-                }
                 }
                 const unsigned cs = curc[self];
                 const unsigned cn = (cs & ~pbit) | (changed_now ? pbit : 0u);
@@ -329,10 +374,10 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
             chist = (chist << 1) | (cchanged ? 1u : 0u);
             any_change |= changed_now;
 
-            // write back the plane every lane has finished (only the nodes that changed)
+            // write back the plane every lane has finished (only what changed)
             const int pw = s - kSkew;
             if (pw >= 0 && pw < na && inb) {
-                const long long g = (long long)node_i(a0 + pw) * plane_nodes + col_off;
+                const long long g = (long long)node_i(a_first + pw) * plane_nodes + col_off;
                 if ((hist >> (kSkew - bb - cc)) & 1u) tt[g] = ring[(pw + 1) % kRing][tblk + self];
                 if ((chist >> (kSkew - bb - cc)) & 1u) cg[g] = ringc[(pw + 1) % kRing][tblk + self];
             }
@@ -346,7 +391,16 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
         }
     }
 
-    if (__any(any_change) && lane == 0 && has_src) A.changed[src] = 1;
+    if (has_src) {
+        const bool wave_changed = __any(any_change);
+        if (lane == 0) {
+            if (wave_changed) A.changed[src] = 1;
+            uint8_t *ts = A.tsum + src * tiles + ((long long)ti * A.nbb + tj) * A.nbc + tk;
+            const unsigned old = *ts;
+            const unsigned nw = (old & ~pbit) | (wave_changed ? pbit : 0u);
+            if (nw != old) *ts = (uint8_t)nw;
+        }
+    }
 }
 
 // tt = Big everywhere (and signs cleared), _fteik3d.py:429-437
@@ -357,6 +411,11 @@ __global__ void fill_kernel(double *tt, long long n, uint8_t *sgn, uint8_t *chg)
         chg[i] = 0;
         if (sgn) sgn[i] = 0;
     }
+}
+
+__global__ void zero_bytes_kernel(uint8_t *p, long long n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
 }
 
 __device__ __forceinline__ double t_ana3(int i, int j, int k, double dz, double dx, double dy,
@@ -511,7 +570,7 @@ void launch_brick(bool grad, int group, dim3 grid, hipStream_t st, const SweepAr
 struct Ws3 {
     SrcParam *prm;
     int *changed, *active, *list, *nlist;
-    uint8_t *chg;
+    uint8_t *chg, *tsum;
 };
 
 Ws3 carve3(void *ws, int nsrc, long long nodes) {
@@ -528,7 +587,8 @@ Ws3 carve3(void *ws, int nsrc, long long nodes) {
     w.nlist = (int *)p;
     p += 256;
     w.chg = (uint8_t *)p;
-    (void)nodes;
+    p += align256((size_t)nodes * nsrc);
+    w.tsum = (uint8_t *)p;
     return w;
 }
 
@@ -539,10 +599,16 @@ using namespace fteik;
 
 extern "C" const char *fteik_version(void) { return "fteik_amd 0.1.0 (gfx950)"; }
 
+// Tiles of the memory grid (z extent tile_a is at least 1): an upper bound on the tile
+// count for any tile_a, used to size the per-source tile bytes.
+static size_t max_tiles(int nz, int nx, int ny) {
+    return (size_t)(nz + 1) * ((nx + 1 + kLanesB - 1) / kLanesB) * ((ny + 1 + kLanesC - 1) / kLanesC);
+}
+
 extern "C" size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc) {
     const size_t nodes = (size_t)(nz + 1) * (nx + 1) * (ny + 1);
     return align256(sizeof(SrcParam) * nsrc) + 3 * align256(sizeof(int) * nsrc) + 256 +
-           align256(nodes * nsrc);
+           align256(nodes * nsrc) + align256(max_tiles(nz, nx, ny) * nsrc);
 }
 
 extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double dz, double dx,
@@ -596,15 +662,22 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
     a.list = w.list;
     a.nlist = w.nlist;
     a.chg = w.chg;
+    a.tsum = w.tsum;
     a.sets_done = sets_done;
     a.node_stride = nn;
     a.nz = nz; a.nx = nx; a.ny = ny;
     a.NZ = NZ; a.NX = NX; a.NY = NY;
-    if (tile_a <= 0) tile_a = nsrc >= 32 ? nz : std::min(nz, 16);
-    a.tile_a = std::min(tile_a, nz);
-    a.nba = ceil_div(nz, a.tile_a);
-    a.nbb = ceil_div(nx, kLanesB);
-    a.nbc = ceil_div(ny, kLanesC);
+    if (tile_a <= 0) tile_a = nsrc >= 32 ? NZ : std::min(NZ, 16);
+    a.tile_a = std::min(tile_a, NZ);
+    a.nba = ceil_div(NZ, a.tile_a);
+    a.nbb = ceil_div(NX, kLanesB);
+    a.nbc = ceil_div(NY, kLanesC);
+    {
+        const long long nt = (long long)a.nba * a.nbb * a.nbc * nsrc;
+        hipLaunchKernelGGL(zero_bytes_kernel, dim3((unsigned)std::min<long long>((nt + 255) / 256, 65536)),
+                           dim3(256), 0, stream, w.tsum, nt);
+        FTEIK_CHECK(hipGetLastError());
+    }
     a.dz = dz; a.dx = dx; a.dy = dy;
     a.dz2i = 1.0 / dz / dz;
     a.dx2i = 1.0 / dx / dx;

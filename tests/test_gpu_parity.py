@@ -205,7 +205,7 @@ def test_many_sets_skip_bitwise(fa):
     eik = fa.Eikonal3D(v, d)
     ref, _, _, kref = oracle.solve3d(1.0 / v, *d, srcs, nsweep=0)
     assert kref.max() >= 10
-    for tile, group in ((0, 4), (16, 1), (8, 2)):
+    for tile, group in ((0, 4), (16, 1), (7, 2), (5, 8), (1, 1)):
         tt, _, _, sets = eik.solve_tensors(srcs, nsweep=None, tile_a=tile, group=group,
                                            use_graph=True)
         assert list(sets) == list(kref), (tile, group)
