@@ -27,14 +27,15 @@ ERR_ENDPOINT_OOB = 2
 ERR_MAX_STEP = 3
 ERR_ARG = 4
 ERR_WORKSPACE = 5
+ERR_TIMEOUT = 6
 
 # name -> (restype, argtypes); every entry point of include/fteik_amd.h
 SIGNATURES = {
     "fteik_version": (ctypes.c_char_p, []),
     "fteik3d_workspace_size": (ctypes.c_size_t, [_c_int, _c_int, _c_int, _c_int]),
     "fteik3d_solve": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _dp, _c_int,
-                               _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp,
-                               ctypes.c_size_t, _c_int, _vp]),
+                               _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                               _vp, ctypes.c_size_t, _c_int, _vp]),
     "fteik3d_gradient": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _dp,
                                   _c_dbl, _vp, _vp]),
     "fteik2d_workspace_size": (ctypes.c_size_t, [_c_int]),
@@ -93,6 +94,8 @@ def check(status):
         raise ValueError("invalid argument")
     if status == ERR_WORKSPACE:
         raise RuntimeError("workspace too small")
+    if status == ERR_TIMEOUT:
+        raise RuntimeError("persistent sweep: a dependency wait gave up (result invalid)")
     raise RuntimeError(f"HIP error {-status}")
 
 

@@ -3,16 +3,32 @@
 // (_fteik3d.py:203-391), with source seeding (_fteik3d.py:394-456) and the gradient
 // post-pass (_fteik3d.py:464-488).
 //
-// Schedule.  Within one directional pass a node reads only itself and 7 upwind neighbours,
-// all on strictly lower anti-diagonal levels of that direction.  Every topological order
-// of a pass therefore yields the reference's state bit for bit (FP contraction is disabled
-// at build time).  The grid is cut into bricks of tile_a (z) x 8 (x) x 8 (y) nodes in the
-// pass's oriented coordinates; bricks on one brick anti-diagonal are independent and are
-// processed by one launch (one 64-lane wave per brick and source).  Inside a brick lane
-// (bb, cc) handles column (x, y) = (b0+bb, c0+cc) and, at step s, plane p = s - bb - cc:
-// all lanes of a step lie on one node hyperplane, so every upwind value was produced at an
-// earlier step.  z planes are staged through an LDS ring with coalesced global loads and
-// written back coalesced once all lanes are past them.
+// Exactness.  Within one directional pass a node reads only itself and 7 upwind
+// neighbours, all on strictly lower anti-diagonal levels of that direction, so every
+// topological order of a pass yields the reference's state bit for bit (FP contraction is
+// disabled at build time).
+//
+// Bricks.  The grid is cut into tiles of tile_a (z) x 8 (x) x 8 (y) nodes fixed in memory
+// coordinates; in each pass a tile is processed as one brick by one 64-lane wave per
+// source.  Lane (bb, cc) owns column (x, y) = (b0+bb, c0+cc) of the pass's oriented
+// coordinates and, at step s, works on plane p = s - bb - cc: all lanes of a step lie on one
+// node hyperplane, so every upwind value was produced at an earlier step.  z planes are
+// staged through an LDS ring with coalesced global loads (4 planes in flight in registers)
+// and written back coalesced once every lane is past them.  W waves of a workgroup run the
+// same brick for W sources and share the staged slowness.
+//
+// Skipping.  Per node, one byte records in which of the last 8 passes the node changed; a
+// node whose 7 upwind neighbours all read zero is skipped (its candidates equal those of
+// its previous update in this direction and its own time can only have decreased).  Per
+// tile, one byte ORs its nodes' bytes; a tile whose byte and its upwind neighbours' bytes
+// are zero is skipped whole.
+//
+// Schedules.  "persistent": one launch per sweep set; resident workgroups take (pass,
+// brick, source group) tasks from a queue in a topological order and wait on per-tile
+// progress counters: the 3 upwind face tiles must have finished this pass and the 3x3x3
+// neighbourhood the previous one (which covers every read-after-write and
+// write-after-read between tiles), so passes overlap across tiles.  "diagonal": one launch
+// per brick anti-diagonal and pass, ordered by the stream.
 #include "fteik_amd.h"
 #include "fteik_device.hpp"
 
@@ -32,6 +48,16 @@ constexpr int kDepth = 4;       // further planes in flight in registers (loop u
 // a slot is rewritten at the end of step s for plane s+kPrefetch; its previous plane was
 // last read (as the upwind plane of lane (7,7)) at step s-1, before that step's barrier
 constexpr int kRing = kPrefetch + kSkew + 2;
+// a dependency wait longer than this many seconds gives up and sets the error flag, so
+// every workgroup exits even if a dependency were never met
+constexpr unsigned long long kWaitSeconds = 10;
+
+// LDS slot pitches: 8 mod 16 (doubles, resp. dwords) spreads the lanes' skewed accesses
+// over the banks.
+__host__ __device__ constexpr int dpitch(int n) { return n + ((8 - n % 16) + 16) % 16; }
+__host__ __device__ constexpr int bpitch(int n) {
+    return 4 * ((n + 3) / 4 + ((8 - ((n + 3) / 4) % 16) + 16) % 16);
+}
 
 struct SweepArgs {
     const double *slow;
@@ -43,16 +69,48 @@ struct SweepArgs {
     const int *list;        // sources still changing (compacted), first *nlist entries
     const int *nlist;
     const int *sets_done;   // sets completed per source (0 during the first set)
-    int pass;               // 0..7, position of this pass in the set
+    int *progress;          // persistent: passes finished per (source group, tile)
+    int *queue;             // persistent: next task
+    int *error;             // persistent: set when a dependency wait gave up
+    const int *cum_nb;      // persistent: bricks on anti-diagonals < d
+    unsigned long long wait_ticks;  // persistent: wall-clock ticks of the wait limit
     long long node_stride;  // nodes per source grid
     int nz, nx, ny;         // cells
     int NZ, NX, NY;         // nodes
-    int sz, sx, sy;         // pass direction (sgnt of the reference)
     int tile_a;             // z extent of a tile / brick
     int nba, nbb, nbc;      // tile counts along z, x, y (memory grid, nodes 0..N-1)
-    int diag;               // brick anti-diagonal of this launch
+    int ntiles, ndiag;
+    int pass;               // diagonal schedule: pass of this launch
+    int diag;               // diagonal schedule: brick anti-diagonal of this launch
     double dz, dx, dy, dz2i, dx2i, dy2i, dz2dx2, dz2dy2, dx2dy2, dsum;
 };
+
+// pass order of the reference (_fteik3d.py:217-391): z flips at pass 4, y at 2, x at 1
+__host__ __device__ __forceinline__ int pass_sz(int p) { return (p & 4) ? -1 : 1; }
+__host__ __device__ __forceinline__ int pass_sx(int p) { return (p & 1) ? -1 : 1; }
+__host__ __device__ __forceinline__ int pass_sy(int p) { return (p & 2) ? -1 : 1; }
+
+// brick (BA, BB, BC) number t of anti-diagonal d, enumerated with BA outermost
+__device__ __forceinline__ void decode_brick(int d, int t, int nba, int nbb, int nbc, int *BA,
+                                             int *BB, int *BC) {
+    const int amin = max(0, d - (nbb - 1) - (nbc - 1));
+    const int amax = min(nba - 1, d);
+    *BA = amin;
+    *BB = 0;
+    *BC = 0;
+    for (int ba = amin; ba <= amax; ++ba) {
+        const int e = d - ba;
+        const int bmin = max(0, e - (nbc - 1)), bmax = min(nbb - 1, e);
+        const int cnt = bmax - bmin + 1;
+        if (t < cnt) {
+            *BA = ba;
+            *BB = bmin + t;
+            *BC = e - *BB;
+            return;
+        }
+        t -= cnt;
+    }
+}
 
 __device__ __forceinline__ double min4(double a, double b, double c, double d) {
     return fmin(fmin(a, b), fmin(c, d));
@@ -138,60 +196,36 @@ __device__ __forceinline__ double node_update(const SweepArgs &A, double t0, dou
     return tnew;
 }
 
-// Each workgroup runs W waves on the same brick for W different sources (from the
-// compacted list of sources still changing): the slowness block of every staged plane is
-// loaded once into LDS and shared, each wave owns the traveltime block of its source.
-//
-// Bricks are the tiles of one fixed grid in memory coordinates (tile_a x 8 x 8 nodes), so
-// every pass visits the same tiles whatever its direction; a per-source byte per tile
-// records in which of the last 8 passes any of its nodes changed.  A tile whose own byte
-// and the bytes of its upwind neighbour tiles are all zero contains only nodes that the
-// node-level rule below would skip, so the whole brick is skipped.
+// One brick for W sources (wave w handles list[g0 + w]).  Every early return is uniform
+// across the workgroup.
 template <bool GRAD, int W>
-__global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A) {
-    constexpr int kEnt = kPlane * (W + 1);  // staged entries per plane: W node blocks + cells
+__device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int g0, int nact) {
     constexpr int kThreads = 64 * W;
-    constexpr int NE = (kEnt + kThreads - 1) / kThreads;
-    const int nact = *A.nlist;
-    const int g0 = blockIdx.y * W;
-    if (g0 >= nact) return;  // block-uniform: every source of this group has converged
+    constexpr int kT = kPlane * W;                           // node entries (W blocks)
+    constexpr int NT = (kT + kThreads - 1) / kThreads;       // = 2
+    constexpr int NS = (kPlane + kThreads - 1) / kThreads;   // 2 for W = 1, else 1
+    constexpr int kPitch = dpitch(kT + kPlane);
+    constexpr int kCPitch = bpitch(kT);
+    __shared__ double ring[kRing][kPitch];
+    __shared__ uint8_t ringc[kRing][kCPitch];
+    __shared__ int wave_on[W];
 
-    // brick (BA, BB, BC) on anti-diagonal A.diag, enumerated with BA outermost
-    int t = blockIdx.x, BA = 0, BB = 0, BC = 0;
-    {
-        const int d = A.diag;
-        const int amin = max(0, d - (A.nbb - 1) - (A.nbc - 1));
-        const int amax = min(A.nba - 1, d);
-        for (int ba = amin; ba <= amax; ++ba) {
-            const int e = d - ba;
-            const int bmin = max(0, e - (A.nbc - 1)), bmax = min(A.nbb - 1, e);
-            const int cnt = bmax - bmin + 1;
-            if (t < cnt) {
-                BA = ba;
-                BB = bmin + t;
-                BC = e - BB;
-                break;
-            }
-            t -= cnt;
-        }
-    }
-
+    const int sz = pass_sz(pass), sx = pass_sx(pass), sy = pass_sy(pass);
     const int NX = A.NX, NY = A.NY;
     // oriented brick origin: tile BA/BB/BC of the memory grid seen from this direction
-    const int a0 = A.tile_a * BA - (A.sz > 0 ? 0 : A.tile_a * A.nba - A.NZ);
-    const int b0 = kLanesB * BB - (A.sx > 0 ? 0 : kLanesB * A.nbb - NX);
-    const int c0 = kLanesC * BC - (A.sy > 0 ? 0 : kLanesC * A.nbc - NY);
+    const int a0 = A.tile_a * BA - (sz > 0 ? 0 : A.tile_a * A.nba - A.NZ);
+    const int b0 = kLanesB * BB - (sx > 0 ? 0 : kLanesB * A.nbb - NX);
+    const int c0 = kLanesC * BC - (sy > 0 ? 0 : kLanesC * A.nbc - NY);
     const int a_first = max(1, a0);
     const int na = min(A.NZ - 1, a0 + A.tile_a - 1) - a_first + 1;  // planes to update
     if (na <= 0 || b0 + kLanesB - 1 < 1 || c0 + kLanesC - 1 < 1) return;
     // tile of the memory grid and its upwind neighbours
-    const int ti = A.sz > 0 ? BA : A.nba - 1 - BA;
-    const int tj = A.sx > 0 ? BB : A.nbb - 1 - BB;
-    const int tk = A.sy > 0 ? BC : A.nbc - 1 - BC;
-    const long long tiles = (long long)A.nba * A.nbb * A.nbc;
+    const int ti = sz > 0 ? BA : A.nba - 1 - BA;
+    const int tj = sx > 0 ? BB : A.nbb - 1 - BB;
+    const int tk = sy > 0 ? BC : A.nbc - 1 - BC;
+    const long long tiles = A.ntiles;
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    __shared__ int wave_on[W];
     {
         const bool has = g0 + wave < nact;
         const int s0 = has ? A.list[g0 + wave] : 0;
@@ -200,9 +234,9 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
             // lanes 0..7 read the tile byte and its 7 upwind neighbours
             unsigned any = 0;
             if (lane < 8) {
-                const int ii = ti - ((lane & 1) ? A.sz : 0);
-                const int jj = tj - ((lane & 2) ? A.sx : 0);
-                const int kk = tk - ((lane & 4) ? A.sy : 0);
+                const int ii = ti - ((lane & 1) ? sz : 0);
+                const int jj = tj - ((lane & 2) ? sx : 0);
+                const int kk = tk - ((lane & 4) ? sy : 0);
                 if (ii >= 0 && ii < A.nba && jj >= 0 && jj < A.nbb && kk >= 0 && kk < A.nbc)
                     any = A.tsum[s0 * tiles + ((long long)ii * A.nbb + jj) * A.nbc + kk];
             }
@@ -214,7 +248,7 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
     bool block_on = false;
 #pragma unroll
     for (int w = 0; w < W; ++w) block_on |= wave_on[w] != 0;
-    if (!block_on) return;  // block-uniform
+    if (!block_on) return;
 
     const bool has_src = wave_on[wave] != 0;
     const int src = has_src ? A.list[g0 + wave] : 0;
@@ -223,124 +257,135 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
     const bool inb = has_src && b >= 1 && b <= NX - 1 && c >= 1 && c <= NY - 1;
 
     // oriented -> memory index maps
-    auto node_i = [&](int a) { return A.sz > 0 ? a : A.NZ - 1 - a; };
-    auto node_j = [&](int x) { return A.sx > 0 ? x : NX - 1 - x; };
-    auto node_k = [&](int y) { return A.sy > 0 ? y : NY - 1 - y; };
-    auto cell_i = [&](int a) { return A.sz > 0 ? a : A.nz - 1 - a; };
-    auto cell_j = [&](int x) { return A.sx > 0 ? x : A.nx - 1 - x; };
-    auto cell_k = [&](int y) { return A.sy > 0 ? y : A.ny - 1 - y; };
+    auto node_i = [&](int a) { return sz > 0 ? a : A.NZ - 1 - a; };
+    auto node_j = [&](int x) { return sx > 0 ? x : NX - 1 - x; };
+    auto node_k = [&](int y) { return sy > 0 ? y : NY - 1 - y; };
+    auto cell_i = [&](int a) { return sz > 0 ? a : A.nz - 1 - a; };
+    auto cell_j = [&](int x) { return sx > 0 ? x : A.nx - 1 - x; };
+    auto cell_k = [&](int y) { return sy > 0 ? y : A.ny - 1 - y; };
 
     double *tt = A.tt + (long long)src * A.node_stride;
     uint8_t *sg = GRAD ? A.sgn + (long long)src * A.node_stride : nullptr;
     const long long plane_nodes = (long long)NX * NY;
     const long long plane_cells = (long long)A.nx * A.ny;
 
-    // Staging entries of this thread: e = tid + 64 W n.  e < 81 W: node (eb, ec) of the
-    // block of wave e / 81; then the 81 cells.  Every entry reads a valid address (nodes
-    // outside the grid read the slowness base and are replaced by Big).
-    const double *ent_base[NE];
-    const uint8_t *ent_chg[NE];
-    bool ent_node[NE], ent_big[NE], ent_use[NE], ent_c[NE];
+    // Staging: node entries e = tid + 64 W n (< 81 W; node (eb, ec) of wave e / 81's
+    // block), cell entries e = tid + 64 W m (< 81).  Nodes outside the grid read as Big.
+    long long t_off[NT];
+    bool t_use[NT], t_ok[NT];
 #pragma unroll
-    for (int n = 0; n < NE; ++n) {
+    for (int n = 0; n < NT; ++n) {
         const int e = tid + kThreads * n;
-        ent_use[n] = e < kEnt;
-        ent_c[n] = e < kPlane * W;
-        ent_node[n] = false;
-        ent_big[n] = false;
-        ent_base[n] = A.slow;
-        ent_chg[n] = A.chg;
-        if (e < kPlane * W) {
+        t_use[n] = e < kT;
+        t_ok[n] = false;
+        t_off[n] = 0;
+        if (e < kT) {
             const int w = e / kPlane, r = e - w * kPlane;
             const int eb = b0 - 1 + r / 9, ec = c0 - 1 + r % 9;
             if (wave_on[w] && eb >= 0 && eb <= NX - 1 && ec >= 0 && ec <= NY - 1) {
-                ent_node[n] = true;
-                const long long off = (long long)A.list[g0 + w] * A.node_stride +
-                                      (long long)node_j(eb) * NY + node_k(ec);
-                ent_base[n] = A.tt + off;
-                ent_chg[n] = A.chg + off;
-            } else {
-                ent_big[n] = true;
+                t_ok[n] = true;
+                t_off[n] = (long long)A.list[g0 + w] * A.node_stride + (long long)node_j(eb) * NY +
+                           node_k(ec);
             }
-        } else if (e < kEnt) {
-            const int r = e - kPlane * W;
-            const int cb = min(max(b0 - 1 + r / 9, 0), A.nx - 1);
-            const int ccl = min(max(c0 - 1 + r % 9, 0), A.ny - 1);
-            ent_base[n] = A.slow + (long long)cell_j(cb) * A.ny + cell_k(ccl);
         }
     }
-
-    __shared__ double ring[kRing][kEnt];
-    __shared__ uint8_t ringc[kRing][kPlane * W];
+    long long s_off[NS];
+    bool s_use[NS];
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+        const int e = tid + kThreads * m;
+        s_use[m] = e < kPlane;
+        const int r = s_use[m] ? e : 0;
+        const int cb = min(max(b0 - 1 + r / 9, 0), A.nx - 1);
+        const int ccl = min(max(c0 - 1 + r % 9, 0), A.ny - 1);
+        s_off[m] = (long long)cell_j(cb) * A.ny + cell_k(ccl);
+    }
 
     // plane p of the brick (a = a_first + p; p = -1 is the upwind plane) lives in slot
     // (p+1) % kRing
-    auto load_plane = [&](int p, double *v, unsigned *cv) {
+    struct Stage {
+        double t[NT];
+        unsigned c[NT];
+        double s[NS];
+    };
+    auto load_plane = [&](int p, Stage &v) {
         const int a = a_first + p;
         const long long tb = (long long)node_i(a) * plane_nodes;
         const long long sb = (long long)cell_i(min(a, A.nz - 1)) * plane_cells;
 #pragma unroll
-        for (int n = 0; n < NE; ++n) {
-            const double x = ent_base[n][ent_node[n] ? tb : sb];
-            v[n] = ent_big[n] ? kBig : x;
-            const unsigned cb = ent_chg[n][ent_node[n] ? tb : 0];
-            cv[n] = ent_node[n] ? cb : 0u;
+        for (int n = 0; n < NT; ++n) {
+            v.t[n] = kBig;
+            v.c[n] = 0u;
+            if (t_ok[n]) {
+                v.t[n] = A.tt[t_off[n] + tb];
+                v.c[n] = A.chg[t_off[n] + tb];
+            }
         }
+#pragma unroll
+        for (int m = 0; m < NS; ++m)
+            if (s_use[m]) v.s[m] = A.slow[s_off[m] + sb];
     };
-    auto store_plane = [&](int p, const double *v, const unsigned *cv) {
+    auto store_plane = [&](int p, const Stage &v) {
         double *slot = ring[(p + 1) % kRing];
         uint8_t *cslot = ringc[(p + 1) % kRing];
 #pragma unroll
-        for (int n = 0; n < NE; ++n) {
-            if (ent_use[n]) slot[tid + kThreads * n] = v[n];
-            if (ent_c[n]) cslot[tid + kThreads * n] = (uint8_t)cv[n];
-        }
+        for (int n = 0; n < NT; ++n)
+            if (t_use[n]) {
+                slot[tid + kThreads * n] = v.t[n];
+                cslot[tid + kThreads * n] = (uint8_t)v.c[n];
+            }
+#pragma unroll
+        for (int m = 0; m < NS; ++m)
+            if (s_use[m]) slot[kT + tid + kThreads * m] = v.s[m];
     };
 
     for (int p = -1; p < min(kPrefetch, na); ++p) {
-        double v[NE];
-        unsigned cv[NE];
-        load_plane(p, v, cv);
-        store_plane(p, v, cv);
+        Stage v;
+        load_plane(p, v);
+        store_plane(p, v);
     }
     // planes kPrefetch .. kPrefetch+kDepth-1 in flight in registers
-    double rv[kDepth][NE];
-    unsigned rc[kDepth][NE];
+    Stage rv[kDepth];
 #pragma unroll
     for (int k = 0; k < kDepth; ++k)
-        if (kPrefetch + k < na) load_plane(kPrefetch + k, rv[k], rc[k]);
+        if (kPrefetch + k < na) load_plane(kPrefetch + k, rv[k]);
     __syncthreads();
     // From the second set on, a node none of whose 7 upwind neighbours changed during the
     // last 8 passes sees exactly the candidates of its previous update in this direction,
     // while its own time can only have decreased since: it cannot change and is skipped.
     const bool may_skip = has_src && A.sets_done[src] > 0;
-    const unsigned pbit = 1u << A.pass;
+    const unsigned pbit = 1u << pass;
     uint8_t *cg = A.chg + (long long)src * A.node_stride;
 
-    const int tblk = wave * kPlane, sblk = W * kPlane;
+    const int tblk = wave * kPlane, sblk = kT;
     const int self = (bb + 1) * 9 + (cc + 1);  // this lane's node in a block
     const int up_b = bb * 9 + (cc + 1);        // b-1
     const int up_c = (bb + 1) * 9 + cc;        // c-1
     const int up_bc = bb * 9 + cc;             // b-1, c-1
     const long long col_off = inb ? (long long)node_j(b) * NY + node_k(c) : 0;
-    const uint8_t code_z = sgn_code(A.sz), code_x = (uint8_t)(sgn_code(A.sx) << 2),
-                  code_y = (uint8_t)(sgn_code(A.sy) << 4);
+    const uint8_t code_z = sgn_code(sz), code_x = (uint8_t)(sgn_code(sx) << 2),
+                  code_y = (uint8_t)(sgn_code(sy) << 4);
 
     unsigned hist = 0;   // bit k: this lane changed its node k steps ago
     unsigned chist = 0;  // bit k: this lane changed its node's change bits k steps ago
     bool any_change = false;
     const int nsteps = na + kSkew;
+    // ring slots advance by one per step: this lane's current plane p = s - bb - cc sits in
+    // slot (p+1) mod kRing, the plane written back at step s in slot (s-kSkew+1) mod kRing
+    int cslot = (1 - bb - cc + 2 * kRing) % kRing;
+    int wslot = (1 - kSkew + 2 * kRing) % kRing;
     for (int s0 = 0; s0 < nsteps; s0 += kDepth) {
 #pragma unroll
         for (int k = 0; k < kDepth; ++k) {
             const int s = s0 + k;
             const int p = s - bb - cc;
             bool changed_now = false, cchanged = false;
+            const int pslot = cslot == 0 ? kRing - 1 : cslot - 1;
             if (inb && p >= 0 && p < na) {
-                double *cur = ring[(p + 1) % kRing];
-                const double *prv = ring[p % kRing];
-                uint8_t *curc = ringc[(p + 1) % kRing] + tblk;
-                const uint8_t *prvc = ringc[p % kRing] + tblk;
+                double *cur = ring[cslot];
+                const double *prv = ring[pslot];
+                uint8_t *curc = ringc[cslot] + tblk;
+                const uint8_t *prvc = ringc[pslot] + tblk;
                 const unsigned up = curc[up_b] | curc[up_c] | curc[up_bc] | prvc[self] |
                                     prvc[up_b] | prvc[up_c] | prvc[up_bc];
                 const double t0 = cur[tblk + self];
@@ -378,15 +423,17 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
             const int pw = s - kSkew;
             if (pw >= 0 && pw < na && inb) {
                 const long long g = (long long)node_i(a_first + pw) * plane_nodes + col_off;
-                if ((hist >> (kSkew - bb - cc)) & 1u) tt[g] = ring[(pw + 1) % kRing][tblk + self];
-                if ((chist >> (kSkew - bb - cc)) & 1u) cg[g] = ringc[(pw + 1) % kRing][tblk + self];
+                if ((hist >> (kSkew - bb - cc)) & 1u) tt[g] = ring[wslot][tblk + self];
+                if ((chist >> (kSkew - bb - cc)) & 1u) cg[g] = ringc[wslot][tblk + self];
             }
+            cslot = cslot == kRing - 1 ? 0 : cslot + 1;
+            wslot = wslot == kRing - 1 ? 0 : wslot + 1;
 
             // plane s+kPrefetch leaves the registers for its slot, whose previous plane was
             // last read one step ago; its register set takes the plane kDepth further on
             const int pl = s + kPrefetch;
-            if (pl < na) store_plane(pl, rv[k], rc[k]);
-            if (pl + kDepth < na) load_plane(pl + kDepth, rv[k], rc[k]);
+            if (pl < na) store_plane(pl, rv[k]);
+            if (pl + kDepth < na) load_plane(pl + kDepth, rv[k]);
             __syncthreads();
         }
     }
@@ -401,6 +448,112 @@ __global__ __launch_bounds__(64 * W) void sweep3d_brick_kernel(const SweepArgs A
             if (nw != old) *ts = (uint8_t)nw;
         }
     }
+}
+
+// Diagonal schedule: one launch per (pass, brick anti-diagonal); blockIdx.x = brick on the
+// diagonal, blockIdx.y = source group.
+template <bool GRAD, int W>
+__global__ __launch_bounds__(64 * W) void sweep3d_diag_kernel(const SweepArgs A) {
+    const int nact = *A.nlist;
+    const int g0 = blockIdx.y * W;
+    if (g0 >= nact) return;
+    int BA, BB, BC;
+    decode_brick(A.diag, blockIdx.x, A.nba, A.nbb, A.nbc, &BA, &BB, &BC);
+    brick<GRAD, W>(A, A.pass, BA, BB, BC, g0, nact);
+}
+
+// Persistent schedule: one launch per sweep set.  Task t (taken in increasing order) is
+// pass p = t / (tiles * G), then anti-diagonal d, brick on it, source group g (G groups of
+// the sources still changing).  Every dependency of a task has a smaller index, and a
+// workgroup only takes a task while running, so the smallest unfinished task can always
+// proceed.  A dependency wait longer than kWaitSeconds sets *A.error and gives up so that
+// every workgroup still exits.
+template <bool GRAD, int W>
+__global__ __launch_bounds__(64 * W) void sweep3d_persistent_kernel(const SweepArgs A) {
+    __shared__ int s_task;
+    const int nact = *A.nlist;
+    const int G = (nact + W - 1) / W;
+    const long long per_pass = (long long)A.ntiles * G;
+    const long long ntasks = 8 * per_pass;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) s_task = atomicAdd(A.queue, 1);
+        __syncthreads();
+        const long long t = s_task;
+        __syncthreads();  // s_task is rewritten only after every thread has read it
+        if (t >= ntasks) break;
+        const int p = (int)(t / per_pass);
+        const long long r = t - p * per_pass;
+        // anti-diagonal d: cum_nb[d] * G <= r < cum_nb[d+1] * G
+        const int rq = (int)(r / G);
+        int lo = 0, hi = A.ndiag - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (A.cum_nb[mid] <= rq) lo = mid;
+            else hi = mid - 1;
+        }
+        const int d = lo;
+        const long long j = r - (long long)A.cum_nb[d] * G;
+        const int i = (int)(j / G), g = (int)(j - (long long)i * G);
+        int BA, BB, BC;
+        decode_brick(d, i, A.nba, A.nbb, A.nbc, &BA, &BB, &BC);
+        const int sz = pass_sz(p), sx = pass_sx(p), sy = pass_sy(p);
+        const int ti = sz > 0 ? BA : A.nba - 1 - BA;
+        const int tj = sx > 0 ? BB : A.nbb - 1 - BB;
+        const int tk = sy > 0 ? BC : A.nbc - 1 - BC;
+        int *prog = A.progress + (long long)g * A.ntiles;
+
+        // wait: lanes 0..26 check the 3x3x3 neighbourhood for pass p-1 done, lanes 27..29
+        // the upwind face tiles for pass p done
+        if (tid < 64) {
+            int need = 0, idx = -1;
+            if (tid < 27) {
+                const int ii = ti + tid % 3 - 1, jj = tj + (tid / 3) % 3 - 1, kk = tk + tid / 9 - 1;
+                if (p > 0 && ii >= 0 && ii < A.nba && jj >= 0 && jj < A.nbb && kk >= 0 &&
+                    kk < A.nbc) {
+                    need = p;
+                    idx = (ii * A.nbb + jj) * A.nbc + kk;
+                }
+            } else if (tid < 30) {
+                const int ax = tid - 27;
+                const int ii = ti - (ax == 0 ? sz : 0), jj = tj - (ax == 1 ? sx : 0),
+                          kk = tk - (ax == 2 ? sy : 0);
+                if (ii >= 0 && ii < A.nba && jj >= 0 && jj < A.nbb && kk >= 0 && kk < A.nbc) {
+                    need = p + 1;
+                    idx = (ii * A.nbb + jj) * A.nbc + kk;
+                }
+            }
+            bool ok = idx < 0;
+            const unsigned long long t_start = wall_clock64();
+            while (!__all(ok)) {
+                if (!ok)
+                    ok = __hip_atomic_load(prog + idx, __ATOMIC_ACQUIRE,
+                                           __HIP_MEMORY_SCOPE_AGENT) >= need;
+                if (__all(ok)) break;
+                if (wall_clock64() - t_start > A.wait_ticks) {
+                    if (tid == 0) atomicExch(A.error, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __syncthreads();
+        __threadfence();  // acquire for every wave: see the producers' data
+        brick<GRAD, W>(A, p, BA, BB, BC, g * W, nact);
+        __threadfence();  // release this thread's writes
+        __syncthreads();
+        if (tid == 0)
+            __hip_atomic_store(prog + (ti * A.nbb + tj) * A.nbc + tk, p + 1, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Start of every set on the persistent schedule: reset queue and progress counters.
+__global__ void set_reset_kernel(int *queue, int *progress, long long n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i0 == 0) *queue = 0;
+    for (long long i = i0; i < n; i += stride) progress[i] = 0;
 }
 
 // tt = Big everywhere (and signs cleared), _fteik3d.py:429-437
@@ -550,46 +703,99 @@ int bricks_on_diag(int d, int nba, int nbb, int nbc) {
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 template <bool G, int W>
-void launch_brick_t(dim3 grid, hipStream_t st, const SweepArgs &a) {
-    hipLaunchKernelGGL((sweep3d_brick_kernel<G, W>), grid, dim3(64 * W), 0, st, a);
+void launch_diag_t(dim3 grid, hipStream_t st, const SweepArgs &a) {
+    hipLaunchKernelGGL((sweep3d_diag_kernel<G, W>), grid, dim3(64 * W), 0, st, a);
 }
 
-void launch_brick(bool grad, int group, dim3 grid, hipStream_t st, const SweepArgs &a) {
+template <bool G, int W>
+void launch_pers_t(int blocks, hipStream_t st, const SweepArgs &a) {
+    hipLaunchKernelGGL((sweep3d_persistent_kernel<G, W>), dim3(blocks), dim3(64 * W), 0, st, a);
+}
+
+#define FTEIK_DISPATCH_W(FN, grad, group, ...)                          \
+    switch ((group) * 2 + ((grad) ? 1 : 0)) {                           \
+        case 2: FN<false, 1>(__VA_ARGS__); break;                       \
+        case 3: FN<true, 1>(__VA_ARGS__); break;                        \
+        case 4: FN<false, 2>(__VA_ARGS__); break;                       \
+        case 5: FN<true, 2>(__VA_ARGS__); break;                        \
+        case 8: FN<false, 4>(__VA_ARGS__); break;                       \
+        case 9: FN<true, 4>(__VA_ARGS__); break;                        \
+        case 16: FN<false, 8>(__VA_ARGS__); break;                      \
+        default: FN<true, 8>(__VA_ARGS__); break;                       \
+    }
+
+template <bool G, int W>
+int pers_blocks_t() {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return -1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, sweep3d_persistent_kernel<G, W>,
+                                                     64 * W, 0) != hipSuccess)
+        return -1;
+    return cus * std::max(per, 1);
+}
+
+int pers_blocks(bool grad, int group) {
     switch (group * 2 + (grad ? 1 : 0)) {
-        case 2: launch_brick_t<false, 1>(grid, st, a); break;
-        case 3: launch_brick_t<true, 1>(grid, st, a); break;
-        case 4: launch_brick_t<false, 2>(grid, st, a); break;
-        case 5: launch_brick_t<true, 2>(grid, st, a); break;
-        case 8: launch_brick_t<false, 4>(grid, st, a); break;
-        case 9: launch_brick_t<true, 4>(grid, st, a); break;
-        case 16: launch_brick_t<false, 8>(grid, st, a); break;
-        default: launch_brick_t<true, 8>(grid, st, a); break;
+        case 2: return pers_blocks_t<false, 1>();
+        case 3: return pers_blocks_t<true, 1>();
+        case 4: return pers_blocks_t<false, 2>();
+        case 5: return pers_blocks_t<true, 2>();
+        case 8: return pers_blocks_t<false, 4>();
+        case 9: return pers_blocks_t<true, 4>();
+        case 16: return pers_blocks_t<false, 8>();
+        default: return pers_blocks_t<true, 8>();
     }
 }
 
 struct Ws3 {
     SrcParam *prm;
-    int *changed, *active, *list, *nlist;
+    int *changed, *active, *list, *nlist, *queue, *error, *cum_nb, *progress;
     uint8_t *chg, *tsum;
 };
 
-Ws3 carve3(void *ws, int nsrc, long long nodes) {
-    char *p = (char *)ws;
-    Ws3 w;
-    w.prm = (SrcParam *)p;
-    p += align256(sizeof(SrcParam) * nsrc);
-    w.changed = (int *)p;
-    p += align256(sizeof(int) * nsrc);
-    w.active = (int *)p;
-    p += align256(sizeof(int) * nsrc);
-    w.list = (int *)p;
-    p += align256(sizeof(int) * nsrc);
-    w.nlist = (int *)p;
-    p += 256;
-    w.chg = (uint8_t *)p;
-    p += align256((size_t)nodes * nsrc);
-    w.tsum = (uint8_t *)p;
-    return w;
+// Upper bounds over every tile_a >= 1: tiles of the memory grid and anti-diagonals.
+size_t max_tiles(int nz, int nx, int ny) {
+    return (size_t)(nz + 1) * ((nx + 1 + kLanesB - 1) / kLanesB) *
+           ((ny + 1 + kLanesC - 1) / kLanesC);
+}
+size_t max_diag(int nz, int nx, int ny) {
+    return (size_t)(nz + 1) + (nx + 1 + kLanesB - 1) / kLanesB + (ny + 1 + kLanesC - 1) / kLanesC;
+}
+
+size_t ws_bytes(int nz, int nx, int ny, int nsrc, Ws3 *w, char *base) {
+    const size_t nodes = (size_t)(nz + 1) * (nx + 1) * (ny + 1);
+    const size_t tiles = max_tiles(nz, nx, ny);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *p = base ? base + off : nullptr;
+        off += align256(bytes);
+        return p;
+    };
+    char *prm = take(sizeof(SrcParam) * nsrc);
+    char *changed = take(sizeof(int) * nsrc);
+    char *active = take(sizeof(int) * nsrc);
+    char *list = take(sizeof(int) * nsrc);
+    char *misc = take(sizeof(int) * 8);  // nlist, queue, error
+    char *cum = take(sizeof(int) * (max_diag(nz, nx, ny) + 1));
+    char *prog = take(sizeof(int) * tiles * nsrc);
+    char *chg = take(nodes * nsrc);
+    char *tsum = take(tiles * nsrc);
+    if (w) {
+        w->prm = (SrcParam *)prm;
+        w->changed = (int *)changed;
+        w->active = (int *)active;
+        w->list = (int *)list;
+        w->nlist = (int *)misc;
+        w->queue = (int *)misc + 1;
+        w->error = (int *)misc + 2;
+        w->cum_nb = (int *)cum;
+        w->progress = (int *)prog;
+        w->chg = (uint8_t *)chg;
+        w->tsum = (uint8_t *)tsum;
+    }
+    return off;
 }
 
 }  // namespace
@@ -597,24 +803,16 @@ Ws3 carve3(void *ws, int nsrc, long long nodes) {
 
 using namespace fteik;
 
-extern "C" const char *fteik_version(void) { return "fteik_amd 0.1.0 (gfx950)"; }
-
-// Tiles of the memory grid (z extent tile_a is at least 1): an upper bound on the tile
-// count for any tile_a, used to size the per-source tile bytes.
-static size_t max_tiles(int nz, int nx, int ny) {
-    return (size_t)(nz + 1) * ((nx + 1 + kLanesB - 1) / kLanesB) * ((ny + 1 + kLanesC - 1) / kLanesC);
-}
+extern "C" const char *fteik_version(void) { return "fteik_amd 0.2.0 (gfx950)"; }
 
 extern "C" size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc) {
-    const size_t nodes = (size_t)(nz + 1) * (nx + 1) * (ny + 1);
-    return align256(sizeof(SrcParam) * nsrc) + 3 * align256(sizeof(int) * nsrc) + 256 +
-           align256(nodes * nsrc) + align256(max_tiles(nz, nx, ny) * nsrc);
+    return ws_bytes(nz, nx, ny, nsrc, nullptr, nullptr);
 }
 
 extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double dz, double dx,
                              double dy, const double *src, int nsrc, int nsweep, int max_sets,
-                             int want_grad, int tile_a, int group, double *tt, uint8_t *sgn,
-                             double *vzero, int *sets_done, void *workspace,
+                             int want_grad, int tile_a, int group, int schedule, double *tt,
+                             uint8_t *sgn, double *vzero, int *sets_done, void *workspace,
                              size_t workspace_bytes, int use_graph, void *stream_) {
     if (nz < 1 || nx < 1 || ny < 1 || nsrc < 1 || nsrc > 65535 || !slow || !src || !tt ||
         !vzero || !sets_done || !workspace)
@@ -625,6 +823,7 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
     if (nsweep <= 0 && max_sets < 1) return FTEIK_ERR_ARG;
     if (group == 0) group = nsrc >= 64 ? 4 : (nsrc >= 8 ? 2 : 1);
     if (group != 1 && group != 2 && group != 4 && group != 8) return FTEIK_ERR_ARG;
+    if (schedule != 0 && schedule != 1) return FTEIK_ERR_ARG;
     hipStream_t stream = (hipStream_t)stream_;
 
     std::vector<SrcParam> prm(nsrc);
@@ -634,36 +833,23 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
 
     const int NZ = nz + 1, NX = nx + 1, NY = ny + 1;
     const long long nn = (long long)NZ * NX * NY;
-    Ws3 w = carve3(workspace, nsrc, nn);
-    FTEIK_CHECK(hipMemcpyAsync(w.prm, prm.data(), sizeof(SrcParam) * nsrc, hipMemcpyHostToDevice,
-                               stream));
-    // the host vector goes out of scope on return: finish the copy first
-    FTEIK_CHECK(hipStreamSynchronize(stream));
-
-    {
-        const long long total = nn * nsrc;
-        const int threads = 256;
-        const long long want = (total + threads - 1) / threads;
-        const int blocks = (int)std::min<long long>(want, 65536);
-        hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(threads), 0, stream, tt, total,
-                           want_grad ? sgn : (uint8_t *)nullptr, w.chg);
-        FTEIK_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(seed3d_kernel, dim3((nsrc * 8 + 255) / 256), dim3(256), 0, stream,
-                           slow, nz, nx, ny, dz, dx, dy, w.prm, nsrc, tt, nn, vzero, sets_done,
-                           w.changed, w.active, w.list, w.nlist);
-        FTEIK_CHECK(hipGetLastError());
-    }
+    Ws3 w;
+    ws_bytes(nz, nx, ny, nsrc, &w, (char *)workspace);
 
     SweepArgs a;
     a.slow = slow;
     a.tt = tt;
     a.sgn = want_grad ? sgn : nullptr;
+    a.chg = w.chg;
+    a.tsum = w.tsum;
     a.changed = w.changed;
     a.list = w.list;
     a.nlist = w.nlist;
-    a.chg = w.chg;
-    a.tsum = w.tsum;
     a.sets_done = sets_done;
+    a.progress = w.progress;
+    a.queue = w.queue;
+    a.error = w.error;
+    a.cum_nb = w.cum_nb;
     a.node_stride = nn;
     a.nz = nz; a.nx = nx; a.ny = ny;
     a.NZ = NZ; a.NX = NX; a.NY = NY;
@@ -672,12 +858,10 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
     a.nba = ceil_div(NZ, a.tile_a);
     a.nbb = ceil_div(NX, kLanesB);
     a.nbc = ceil_div(NY, kLanesC);
-    {
-        const long long nt = (long long)a.nba * a.nbb * a.nbc * nsrc;
-        hipLaunchKernelGGL(zero_bytes_kernel, dim3((unsigned)std::min<long long>((nt + 255) / 256, 65536)),
-                           dim3(256), 0, stream, w.tsum, nt);
-        FTEIK_CHECK(hipGetLastError());
-    }
+    a.ntiles = a.nba * a.nbb * a.nbc;
+    a.ndiag = a.nba + a.nbb + a.nbc - 2;
+    a.pass = 0;
+    a.diag = 0;
     a.dz = dz; a.dx = dx; a.dy = dy;
     a.dz2i = 1.0 / dz / dz;
     a.dx2i = 1.0 / dx / dx;
@@ -686,22 +870,62 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
     a.dz2dy2 = a.dz2i * a.dy2i;
     a.dx2dy2 = a.dx2i * a.dy2i;
     a.dsum = a.dz2i + a.dx2i + a.dy2i;
-    const int ndiag = a.nba + a.nbb + a.nbc - 2;
-    std::vector<int> nb(ndiag);
-    for (int d = 0; d < ndiag; ++d) nb[d] = bricks_on_diag(d, a.nba, a.nbb, a.nbc);
+    std::vector<int> nb(a.ndiag), cum(a.ndiag + 1, 0);
+    for (int d = 0; d < a.ndiag; ++d) {
+        nb[d] = bricks_on_diag(d, a.nba, a.nbb, a.nbc);
+        cum[d + 1] = cum[d] + nb[d];
+    }
+    const long long gmax = ceil_div(nsrc, group);
+    if (8LL * a.ntiles * gmax >= (1LL << 31)) return FTEIK_ERR_ARG;  // task index is an int
+    const int pblocks = schedule == 0 ? pers_blocks(want_grad != 0, group) : 0;
+    if (schedule == 0 && pblocks <= 0) return FTEIK_ERR_ARG;
+    {
+        int dev = 0, khz = 0;
+        FTEIK_CHECK(hipGetDevice(&dev));
+        FTEIK_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        a.wait_ticks = (unsigned long long)std::max(khz, 1) * 1000ull * kWaitSeconds;
+    }
 
-    // pass order of the reference, _fteik3d.py:217-391
-    static const int dirs[8][3] = {{1, 1, 1},  {1, -1, 1},  {1, 1, -1},  {1, -1, -1},
-                                   {-1, 1, 1}, {-1, -1, 1}, {-1, 1, -1}, {-1, -1, -1}};
+    FTEIK_CHECK(hipMemcpyAsync(w.prm, prm.data(), sizeof(SrcParam) * nsrc, hipMemcpyHostToDevice,
+                               stream));
+    FTEIK_CHECK(hipMemcpyAsync(w.cum_nb, cum.data(), sizeof(int) * (a.ndiag + 1),
+                               hipMemcpyHostToDevice, stream));
+    // the host vectors go out of scope on return: finish the copies first
+    FTEIK_CHECK(hipStreamSynchronize(stream));
+    {
+        const long long total = nn * nsrc;
+        const int blocks = (int)std::min<long long>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, stream, tt, total,
+                           want_grad ? sgn : (uint8_t *)nullptr, w.chg);
+        FTEIK_CHECK(hipGetLastError());
+        const long long nt = (long long)a.ntiles * nsrc;
+        hipLaunchKernelGGL(zero_bytes_kernel, dim3((unsigned)std::min<long long>((nt + 255) / 256, 65536)),
+                           dim3(256), 0, stream, w.tsum, nt);
+        FTEIK_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(seed3d_kernel, dim3((nsrc * 8 + 255) / 256), dim3(256), 0, stream,
+                           slow, nz, nx, ny, dz, dx, dy, w.prm, nsrc, tt, nn, vzero, sets_done,
+                           w.changed, w.active, w.list, w.nlist);
+        FTEIK_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(zero_bytes_kernel, dim3(1), dim3(64), 0, stream, (uint8_t *)w.error,
+                           (long long)sizeof(int));
+        FTEIK_CHECK(hipGetLastError());
+    }
+
     auto enqueue_set = [&]() -> int {
-        for (int p = 0; p < 8; ++p) {
-            a.sz = dirs[p][0];
-            a.sx = dirs[p][1];
-            a.sy = dirs[p][2];
-            a.pass = p;
-            for (int d = 0; d < ndiag; ++d) {
-                a.diag = d;
-                launch_brick(want_grad != 0, group, dim3(nb[d], ceil_div(nsrc, group)), stream, a);
+        if (schedule == 0) {
+            const long long np = (long long)a.ntiles * gmax;
+            hipLaunchKernelGGL(set_reset_kernel,
+                               dim3((unsigned)std::min<long long>((np + 255) / 256, 4096)), dim3(256),
+                               0, stream, w.queue, w.progress, np);
+            FTEIK_DISPATCH_W(launch_pers_t, want_grad != 0, group, pblocks, stream, a);
+        } else {
+            for (int p = 0; p < 8; ++p) {
+                a.pass = p;
+                for (int d = 0; d < a.ndiag; ++d) {
+                    a.diag = d;
+                    FTEIK_DISPATCH_W(launch_diag_t, want_grad != 0, group,
+                                     dim3(nb[d], (unsigned)gmax), stream, a);
+                }
             }
         }
         hipLaunchKernelGGL(set_bookkeeping_kernel, dim3(1), dim3(256), 0, stream, nsrc, sets_done,
@@ -712,8 +936,7 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     if (use_graph && stream) {  // the legacy default stream cannot be captured
-        hipStreamCaptureMode mode = hipStreamCaptureModeThreadLocal;
-        FTEIK_CHECK(hipStreamBeginCapture(stream, mode));
+        FTEIK_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         int st = enqueue_set();
         hipError_t e = hipStreamEndCapture(stream, &graph);
         if (st) return st;
@@ -740,6 +963,14 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
     }
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (!status && schedule == 0) {
+        // a dependency wait that gave up leaves a wrong result: report it
+        int err = 0;
+        hipError_t e = hipMemcpyAsync(&err, w.error, sizeof(int), hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        status = hip_status(e);
+        if (!status && err) status = FTEIK_ERR_TIMEOUT;
+    }
     return status;
 }
 

@@ -12,7 +12,8 @@
  * (nz, nx[, ny]) C-ordered FP64, traveltime is node based (nz+1, nx+1[, ny+1]).
  *
  * Return codes: 0 OK; 1 source out of bound; 2 end point out of bound; 3 maximum number
- * of steps reached; 4 invalid argument; 5 workspace too small; negative: -(hipError_t).
+ * of steps reached; 4 invalid argument; 5 workspace too small; 6 a dependency wait of the
+ * persistent sweep gave up (result invalid); negative: -(hipError_t).
  */
 #ifndef FTEIK_AMD_H
 #define FTEIK_AMD_H
@@ -30,6 +31,7 @@ extern "C" {
 #define FTEIK_ERR_MAX_STEP 3
 #define FTEIK_ERR_ARG 4
 #define FTEIK_ERR_WORKSPACE 5
+#define FTEIK_ERR_TIMEOUT 6
 
 /* Library version string. */
 const char *fteik_version(void);
@@ -53,6 +55,8 @@ size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc);
  * want_grad record the upwind direction of every update (needed by fteik3d_gradient)
  * tile_a    nodes per brick along z (0 = library default for the shape)
  * group     sources per workgroup sharing the staged slowness (0 = default; 1, 2, 4, 8)
+ * schedule  0: persistent (one launch per sweep set, tasks wait on per-tile progress);
+ *           1: one launch per (pass, brick anti-diagonal)
  * tt        device [nsrc, nz+1, nx+1, ny+1] traveltimes (output)
  * sgn       device [nsrc, nz+1, nx+1, ny+1] uint8 packed upwind signs or NULL
  * vzero     device [nsrc] slowness of each source cell (output)
@@ -61,12 +65,14 @@ size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc);
  * use_graph capture one sweep set as a hipGraph and replay it (1) or launch directly (0)
  * Checks all sources before enqueuing anything: returns 1 ("source out of bound") with no
  * partial output, as the reference raises (fteikpy/_fteik/_fteik3d.py:402-407).
- * In converge mode (nsweep <= 0) the call synchronizes the stream once per set.
+ * In converge mode (nsweep <= 0) the call synchronizes the stream once per set; the
+ * persistent schedule synchronizes once at the end to read its error flag.
  */
 int fteik3d_solve(const double *slow, int nz, int nx, int ny, double dz, double dx, double dy,
                   const double *src, int nsrc, int nsweep, int max_sets, int want_grad,
-                  int tile_a, int group, double *tt, uint8_t *sgn, double *vzero, int *sets_done,
-                  void *workspace, size_t workspace_bytes, int use_graph, void *stream);
+                  int tile_a, int group, int schedule, double *tt, uint8_t *sgn, double *vzero,
+                  int *sets_done, void *workspace, size_t workspace_bytes, int use_graph,
+                  void *stream);
 
 /*
  * Replaces the gradient post-pass of fteik3d (fteikpy/_fteik/_fteik3d.py:464-488) that

@@ -46,7 +46,7 @@ def test_host_argument_checks():
     src = np.zeros(3)
     sp = src.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     # null slowness pointer
-    st = L.fteik3d_solve(None, 4, 4, 4, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, 0, None, None, None,
+    st = L.fteik3d_solve(None, 4, 4, 4, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, 0, 0, None, None, None,
                          None, None, 0, 0, None)
     assert st == _lib.ERR_ARG
     with pytest.raises(ValueError, match="invalid argument"):
@@ -68,7 +68,7 @@ def test_source_check_before_any_device_work():
     src = np.array([-1.0, 0.0, 0.0])
     sp = src.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     ws = L.fteik3d_workspace_size(4, 4, 4, 1)
-    st = L.fteik3d_solve(fake, 4, 4, 4, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, 0, fake, None, fake, fake,
+    st = L.fteik3d_solve(fake, 4, 4, 4, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, 0, 0, fake, None, fake, fake,
                          fake, ws, 0, None)
     assert st == _lib.ERR_SOURCE_OOB
     src2 = np.array([0.0, 4.5])

@@ -96,8 +96,10 @@ def _rough_model(shape, seed):
     return 1500.0 * np.exp(0.8 * p / np.abs(p).max())
 
 
-@pytest.mark.parametrize("tile_a, group", [(1, 1), (3, 2), (8, 4), (0, 0), (64, 8), (5, 4)])
-def test_brick_shapes_bitwise(fa, tile_a, group):
+@pytest.mark.parametrize("tile_a, group, schedule", [
+    (1, 1, "persistent"), (3, 2, "persistent"), (8, 4, "persistent"), (0, 0, "persistent"),
+    (64, 8, "persistent"), (5, 4, "persistent"), (3, 2, "diagonal"), (0, 4, "diagonal")])
+def test_brick_shapes_bitwise(fa, tile_a, group, schedule):
     """Partial bricks in every axis, several z brick heights and source groups (groups
     partly empty), sources converging after different numbers of sets."""
     shape, d = (21, 19, 27), (7.0, 9.0, 8.0)
@@ -106,7 +108,7 @@ def test_brick_shapes_bitwise(fa, tile_a, group):
                      [70.0, 85.5, 0.0], [0.0, 0.0, 0.0]])
     ref, _, _, kref = oracle.solve3d(1.0 / v, *d, srcs, nsweep=0)
     tt, _, _, sets = fa.Eikonal3D(v, d).solve_tensors(srcs, nsweep=None, tile_a=tile_a,
-                                                      group=group)
+                                                      group=group, schedule=schedule)
     assert list(sets) == list(kref)
     assert np.array_equal(tt.cpu().numpy(), ref)
 
@@ -205,9 +207,10 @@ def test_many_sets_skip_bitwise(fa):
     eik = fa.Eikonal3D(v, d)
     ref, _, _, kref = oracle.solve3d(1.0 / v, *d, srcs, nsweep=0)
     assert kref.max() >= 10
-    for tile, group in ((0, 4), (16, 1), (7, 2), (5, 8), (1, 1)):
+    for tile, group, sched in ((0, 4, "persistent"), (16, 1, "persistent"), (7, 2, "diagonal"),
+                               (5, 8, "persistent"), (1, 1, "persistent")):
         tt, _, _, sets = eik.solve_tensors(srcs, nsweep=None, tile_a=tile, group=group,
-                                           use_graph=True)
+                                           schedule=sched, use_graph=True)
         assert list(sets) == list(kref), (tile, group)
         assert np.array_equal(tt.cpu().numpy(), ref), (tile, group)
     for ns in (5, 9):
