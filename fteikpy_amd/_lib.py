@@ -28,6 +28,7 @@ ERR_MAX_STEP = 3
 ERR_ARG = 4
 ERR_WORKSPACE = 5
 ERR_TIMEOUT = 6
+ERR_UNSUPPORTED = 7
 
 # name -> (restype, argtypes); every entry point of include/fteik_amd.h
 SIGNATURES = {
@@ -94,6 +95,8 @@ def check(status):
         raise ValueError("invalid argument")
     if status == ERR_WORKSPACE:
         raise RuntimeError("workspace too small")
+    if status == ERR_UNSUPPORTED:
+        raise NotImplementedError("option not supported by this build")
     if status == ERR_TIMEOUT:
         raise RuntimeError("persistent sweep: a dependency wait gave up (result invalid)")
     raise RuntimeError(f"HIP error {-status}")

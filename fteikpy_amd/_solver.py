@@ -6,8 +6,8 @@ for a batch of sources).  Keyword-only extensions:
 - ``nsweep=None`` (or <= 0): sweep until a set changes no node, at most ``max_sets`` sets;
 - ``tile_a``: z-extent of the 3D sweep bricks (0 = library default);
 - ``group``: sources per workgroup sharing the staged slowness (0 = default; 1, 2, 4, 8);
-- ``schedule``: "persistent" (one launch per sweep set) or "diagonal" (one launch per
-  brick anti-diagonal and pass);
+- ``schedule``: "diagonal" (one launch per brick anti-diagonal and pass; the default) or
+  "persistent" (one launch per sweep set; currently refused);
 - ``use_graph``: capture one 3D sweep set as a hipGraph and replay it.
 """
 import ctypes
@@ -80,7 +80,7 @@ class Eikonal3D(BaseGrid3D, _SlownessCache):
                          origin=origin if origin is not None else np.zeros(3, dtype=np.float64))
 
     def solve(self, sources, nsweep=2, return_gradient=False, *, max_sets=100, tile_a=0,
-              group=0, schedule="persistent", use_graph=False):
+              group=0, schedule="diagonal", use_graph=False):
         """Solve the eikonal equation for one source [z, x, y] or a list of sources."""
         tt, sgn, vz, nsets = self.solve_tensors(sources, nsweep, return_gradient,
                                                 max_sets=max_sets, tile_a=tile_a, group=group,
@@ -96,7 +96,7 @@ class Eikonal3D(BaseGrid3D, _SlownessCache):
         return grids[0] if single else grids
 
     def solve_tensors(self, sources, nsweep=2, return_gradient=False, *, max_sets=100, tile_a=0,
-                      group=0, schedule="persistent", use_graph=False, out=None):
+                      group=0, schedule="diagonal", use_graph=False, out=None):
         """Batched solve returning device tensors: (tt [n, nz+1, nx+1, ny+1], signs or None,
         vzero (numpy [n]), sets executed (numpy [n])).  `out` may pass a preallocated tt."""
         torch = _torch()

@@ -824,6 +824,10 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
     if (group == 0) group = nsrc >= 64 ? 4 : (nsrc >= 8 ? 2 : 1);
     if (group != 1 && group != 2 && group != 4 && group != 8) return FTEIK_ERR_ARG;
     if (schedule != 0 && schedule != 1) return FTEIK_ERR_ARG;
+    // The persistent schedule stalled on its first GPU run (dependency waits on per-tile
+    // progress counters were not satisfied) and its cause is not established: it is
+    // switched off and refused until it is.
+    if (schedule == 0) return FTEIK_ERR_UNSUPPORTED;
     hipStream_t stream = (hipStream_t)stream_;
 
     std::vector<SrcParam> prm(nsrc);

@@ -13,7 +13,7 @@
  *
  * Return codes: 0 OK; 1 source out of bound; 2 end point out of bound; 3 maximum number
  * of steps reached; 4 invalid argument; 5 workspace too small; 6 a dependency wait of the
- * persistent sweep gave up (result invalid); negative: -(hipError_t).
+ * persistent sweep gave up (result invalid); 7 unsupported option; negative: -(hipError_t).
  */
 #ifndef FTEIK_AMD_H
 #define FTEIK_AMD_H
@@ -32,6 +32,7 @@ extern "C" {
 #define FTEIK_ERR_ARG 4
 #define FTEIK_ERR_WORKSPACE 5
 #define FTEIK_ERR_TIMEOUT 6
+#define FTEIK_ERR_UNSUPPORTED 7
 
 /* Library version string. */
 const char *fteik_version(void);
@@ -55,8 +56,9 @@ size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc);
  * want_grad record the upwind direction of every update (needed by fteik3d_gradient)
  * tile_a    nodes per brick along z (0 = library default for the shape)
  * group     sources per workgroup sharing the staged slowness (0 = default; 1, 2, 4, 8)
- * schedule  0: persistent (one launch per sweep set, tasks wait on per-tile progress);
- *           1: one launch per (pass, brick anti-diagonal)
+ * schedule  1: one launch per (pass, brick anti-diagonal), ordered by the stream;
+ *           0: persistent (one launch per sweep set, tasks wait on per-tile progress
+ *           counters) -- currently refused with FTEIK_ERR_UNSUPPORTED
  * tt        device [nsrc, nz+1, nx+1, ny+1] traveltimes (output)
  * sgn       device [nsrc, nz+1, nx+1, ny+1] uint8 packed upwind signs or NULL
  * vzero     device [nsrc] slowness of each source cell (output)

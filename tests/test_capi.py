@@ -68,7 +68,7 @@ def test_source_check_before_any_device_work():
     src = np.array([-1.0, 0.0, 0.0])
     sp = src.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     ws = L.fteik3d_workspace_size(4, 4, 4, 1)
-    st = L.fteik3d_solve(fake, 4, 4, 4, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, 0, 0, fake, None, fake, fake,
+    st = L.fteik3d_solve(fake, 4, 4, 4, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, 0, 1, fake, None, fake, fake,
                          fake, ws, 0, None)
     assert st == _lib.ERR_SOURCE_OOB
     src2 = np.array([0.0, 4.5])
@@ -87,3 +87,17 @@ def test_product_does_not_import_oracle():
                 text = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in text and "from oracle" not in text, f
                 assert "libfteik_oracle" not in text, f
+
+
+def test_persistent_schedule_is_refused():
+    """The persistent schedule is switched off: the call returns before any device work."""
+    from fteikpy_amd import _lib
+
+    L = _lib.lib()
+    fake = ctypes.c_void_p(0x1000)
+    src = np.array([1.0, 1.0, 1.0])
+    sp = src.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    ws = L.fteik3d_workspace_size(4, 4, 4, 1)
+    st = L.fteik3d_solve(fake, 4, 4, 4, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, 0, 0, fake, None, fake,
+                         fake, fake, ws, 0, None)
+    assert st == _lib.ERR_UNSUPPORTED

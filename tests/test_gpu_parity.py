@@ -96,9 +96,16 @@ def _rough_model(shape, seed):
     return 1500.0 * np.exp(0.8 * p / np.abs(p).max())
 
 
+PERSISTENT_OFF = pytest.mark.skip(
+    reason="persistent schedule stalled on its first GPU run; cause not established, "
+           "switched off in the library (FTEIK_ERR_UNSUPPORTED)")
+
+
 @pytest.mark.parametrize("tile_a, group, schedule", [
-    (1, 1, "persistent"), (3, 2, "persistent"), (8, 4, "persistent"), (0, 0, "persistent"),
-    (64, 8, "persistent"), (5, 4, "persistent"), (3, 2, "diagonal"), (0, 4, "diagonal")])
+    (1, 1, "diagonal"), (3, 2, "diagonal"), (8, 4, "diagonal"), (0, 0, "diagonal"),
+    (64, 8, "diagonal"), (5, 4, "diagonal"),
+    pytest.param(3, 2, "persistent", marks=PERSISTENT_OFF),
+    pytest.param(0, 4, "persistent", marks=PERSISTENT_OFF)])
 def test_brick_shapes_bitwise(fa, tile_a, group, schedule):
     """Partial bricks in every axis, several z brick heights and source groups (groups
     partly empty), sources converging after different numbers of sets."""
@@ -207,8 +214,8 @@ def test_many_sets_skip_bitwise(fa):
     eik = fa.Eikonal3D(v, d)
     ref, _, _, kref = oracle.solve3d(1.0 / v, *d, srcs, nsweep=0)
     assert kref.max() >= 10
-    for tile, group, sched in ((0, 4, "persistent"), (16, 1, "persistent"), (7, 2, "diagonal"),
-                               (5, 8, "persistent"), (1, 1, "persistent")):
+    for tile, group, sched in ((0, 4, "diagonal"), (16, 1, "diagonal"), (7, 2, "diagonal"),
+                               (5, 8, "diagonal"), (1, 1, "diagonal")):
         tt, _, _, sets = eik.solve_tensors(srcs, nsweep=None, tile_a=tile, group=group,
                                            schedule=sched, use_graph=True)
         assert list(sets) == list(kref), (tile, group)
