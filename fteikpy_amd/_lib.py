@@ -29,6 +29,7 @@ ERR_ARG = 4
 ERR_WORKSPACE = 5
 ERR_TIMEOUT = 6
 ERR_UNSUPPORTED = 7
+ERR_UNSETTLED = 8
 
 # name -> (restype, argtypes); every entry point of include/fteik_amd.h
 SIGNATURES = {
@@ -39,6 +40,20 @@ SIGNATURES = {
                                _vp, ctypes.c_size_t, _c_int, _vp]),
     "fteik3d_gradient": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _dp,
                                   _c_dbl, _vp, _vp]),
+    "fteik3d_history_size": (ctypes.c_size_t, [ctypes.c_longlong, _c_int]),
+    "fteik3d_solve_logged": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _dp,
+                                      _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                                      ctypes.c_size_t, _vp, ctypes.c_size_t,
+                                      ctypes.POINTER(ctypes.c_longlong), _c_int, _vp]),
+    "fteik3d_adjoint_history_workspace_size": (ctypes.c_size_t, [_c_int, _c_int, _c_int, _c_int,
+                                                                 ctypes.c_longlong]),
+    "fteik3d_adjoint_history": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _dp,
+                                         _dp, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp,
+                                         ctypes.c_size_t, _vp]),
+    "fteik3d_adjoint_workspace_size": (ctypes.c_size_t, [_c_int, _c_int, _c_int, _c_int]),
+    "fteik3d_adjoint": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _dp, _dp,
+                                 _c_int, _vp, _vp, _c_int, _vp, _vp, ctypes.POINTER(_c_int),
+                                 ctypes.POINTER(_c_int), _vp, ctypes.c_size_t, _vp]),
     "fteik2d_workspace_size": (ctypes.c_size_t, [_c_int]),
     "fteik2d_solve": (_c_int, [_vp, _c_int, _c_int, _c_dbl, _c_dbl, _dp, _c_int, _c_int, _c_int,
                                _c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
@@ -97,6 +112,9 @@ def check(status):
         raise RuntimeError("workspace too small")
     if status == ERR_UNSUPPORTED:
         raise NotImplementedError("option not supported by this build")
+    if status == ERR_UNSETTLED:
+        raise RuntimeError("adjoint: the upwind stencils do not reproduce the solved times "
+                           "(cycle or inconsistent history)")
     if status == ERR_TIMEOUT:
         raise RuntimeError("persistent sweep: a dependency wait gave up (result invalid)")
     raise RuntimeError(f"HIP error {-status}")

@@ -31,6 +31,8 @@
 // per brick anti-diagonal and pass, ordered by the stream.
 #include "fteik_amd.h"
 #include "fteik_device.hpp"
+#include "node_update.hpp"
+#include "history.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -82,6 +84,10 @@ struct SweepArgs {
     int ntiles, ndiag;
     int pass;               // diagonal schedule: pass of this launch
     int diag;               // diagonal schedule: brick anti-diagonal of this launch
+    unsigned *log_node;     // logged solve: changed node (source * nodes + node) ...
+    double *log_old;        // ... and its time before the change
+    unsigned long long *log_count;
+    long long log_cap;
     double dz, dx, dy, dz2i, dx2i, dy2i, dz2dx2, dz2dy2, dx2dy2, dsum;
 };
 
@@ -112,93 +118,9 @@ __device__ __forceinline__ void decode_brick(int d, int t, int nba, int nbb, int
     }
 }
 
-__device__ __forceinline__ double min4(double a, double b, double c, double d) {
-    return fmin(fmin(a, b), fmin(c, d));
-}
-
-// FTeik 3D node update in oriented coordinates (a = z, b = x, c = y, all pointing
-// downwind).  tv/te/tn: upwind neighbours along z/x/y; tev, ten, tnv, tnve: the
-// diagonal ones; sXYZ: slowness of the cell at (a-1 | a') x (b-1 | b') x (c-1 | c'),
-// with ' the downwind-side cell clamped to the grid (the reference's max(i-1,0) /
-// min(i, n-2) pair).  Returns the new value; *which receives the winning candidate.
-__device__ __forceinline__ double node_update(const SweepArgs &A, double t0, double tv, double te,
-                                              double tn, double tev, double ten, double tnv,
-                                              double tnve, double s000, double s001, double s010,
-                                              double s011, double s100, double s101, double s110,
-                                              int *which) {
-    const double dz = A.dz, dx = A.dx, dy = A.dy;
-    const double dz2i = A.dz2i, dx2i = A.dx2i, dy2i = A.dy2i;
-
-    // 1D operators (refracted times), _fteik3d.py:78-104
-    const double t1d1 = tv + dz * min4(s000, s001, s010, s011);
-    const double t1d2 = te + dx * min4(s000, s100, s001, s101);
-    const double t1d3 = tn + dy * min4(s000, s010, s100, s110);
-    const double t1d = fmin(fmin(t1d1, t1d2), t1d3);
-
-    // 2D plane-wave operators, _fteik3d.py:107-143
-    double t2d1 = kBig, t2d2 = kBig, t2d3 = kBig;
-    {
-        const double vref = fmin(s000, s001);  // ZX plane
-        if (tv < te + dx * vref && te < tv + dz * vref) {
-            const double ta = tev + te - tv, tb = tev - te + tv, q = ta - tb;
-            t2d1 = ((tb * dz2i + ta * dx2i) +
-                    sqrt(4.0 * (vref * vref) * (dz2i + dx2i) - dz2i * dx2i * (q * q))) /
-                   (dz2i + dx2i);
-        }
-    }
-    {
-        const double vref = fmin(s000, s010);  // ZY plane
-        if (tv < tn + dy * vref && tn < tv + dz * vref) {
-            const double ta = tv - tn + tnv, tb = tn - tv + tnv, q = ta - tb;
-            t2d2 = ((ta * dz2i + tb * dy2i) +
-                    sqrt(4.0 * (vref * vref) * (dz2i + dy2i) - dz2i * dy2i * (q * q))) /
-                   (dz2i + dy2i);
-        }
-    }
-    {
-        const double vref = fmin(s000, s100);  // XY plane
-        if (te < tn + dy * vref && tn < te + dx * vref) {
-            const double ta = te - tn + ten, tb = tn - te + ten, q = ta - tb;
-            t2d3 = ((ta * dx2i + tb * dy2i) +
-                    sqrt(4.0 * (vref * vref) * (dx2i + dy2i) - dx2i * dy2i * (q * q))) /
-                   (dx2i + dy2i);
-        }
-    }
-    const double t2d = fmin(fmin(t2d1, t2d2), t2d3);
-
-    // 3D operator, _fteik3d.py:146-159
-    double t3d = kBig;
-    if (fmin(t1d, t2d) > fmax(fmax(tv, te), tn)) {
-        const double vref = s000;
-        const double ta = te - 0.5 * tn + 0.5 * ten - 0.5 * tv + 0.5 * tev - tnv + tnve;
-        const double tb = tv - 0.5 * tn + 0.5 * tnv - 0.5 * te + 0.5 * tev - ten + tnve;
-        const double tc = tn - 0.5 * te + 0.5 * ten - 0.5 * tv + 0.5 * tnv - tev + tnve;
-        const double t2 = vref * vref * A.dsum * 9.0;
-        const double q1 = ta - tb, q2 = tb - tc, q3 = ta - tc;
-        double t3 = A.dz2dx2 * (q1 * q1);
-        t3 += A.dz2dy2 * (q2 * q2);
-        t3 += A.dx2dy2 * (q3 * q3);
-        if (t2 >= t3) t3d = (tb * dz2i + ta * dx2i + tc * dy2i + sqrt(t2 - t3)) / A.dsum;
-    }
-
-    const double tnew = fmin(fmin(fmin(t0, t1d), t2d), t3d);
-    // winning candidate in the reference's priority order, _fteik3d.py:166-200;
-    // bit 0 = z, bit 1 = x, bit 2 = y take part
-    int w;
-    if (tnew == t1d1) w = 1;
-    else if (tnew == t1d2) w = 2;
-    else if (tnew == t1d3) w = 4;
-    else if (tnew == t2d1) w = 3;
-    else if (tnew == t2d2) w = 5;
-    else if (tnew == t2d3) w = 6;
-    else w = 7;
-    *which = w;
-    return tnew;
-}
-
 // One brick for W sources (wave w handles list[g0 + w]).  Every early return is uniform
 // across the workgroup.
-template <bool GRAD, int W>
+template <bool GRAD, bool LOG, int W>
 __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int g0, int nact) {
     constexpr int kThreads = 64 * W;
     constexpr int kT = kPlane * W;                           // node entries (W blocks)
@@ -380,6 +302,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
             const int s = s0 + k;
             const int p = s - bb - cc;
             bool changed_now = false, cchanged = false;
+            double t_prev = 0.0;  // LOG: time before the change
             const int pslot = cslot == 0 ? kRing - 1 : cslot - 1;
             if (inb && p >= 0 && p < na) {
                 double *cur = ring[cslot];
@@ -399,6 +322,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
                     cur[tblk + self] = tnew;
                     if (tnew != t0) {
                         changed_now = true;
+                        t_prev = t0;
                         if (GRAD) {
                             uint8_t code = 0;
                             if (which & 1) code |= code_z;
@@ -418,6 +342,27 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
             hist = (hist << 1) | (changed_now ? 1u : 0u);
             chist = (chist << 1) | (cchanged ? 1u : 0u);
             any_change |= changed_now;
+            if (LOG) {
+                // append this wave's changes: one counter update per wave, entries in lane
+                // order
+                const unsigned long long m = __ballot(changed_now);
+                if (m) {
+                    const int leader = __ffsll((long long)m) - 1;
+                    unsigned long long base = 0;
+                    if (lane == leader) base = atomicAdd(A.log_count, (unsigned long long)__popcll(m));
+                    base = __shfl(base, leader);
+                    if (changed_now) {
+                        const unsigned long long e =
+                            base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+                        if (e < (unsigned long long)A.log_cap) {
+                            A.log_node[e] = (unsigned)((long long)src * A.node_stride +
+                                                       (long long)node_i(a_first + p) * plane_nodes +
+                                                       col_off);
+                            A.log_old[e] = t_prev;
+                        }
+                    }
+                }
+            }
 
             // write back the plane every lane has finished (only what changed)
             const int pw = s - kSkew;
@@ -452,14 +397,22 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
 
 // Diagonal schedule: one launch per (pass, brick anti-diagonal); blockIdx.x = brick on the
 // diagonal, blockIdx.y = source group.
-template <bool GRAD, int W>
+template <bool GRAD, bool LOG, int W>
 __global__ __launch_bounds__(64 * W) void sweep3d_diag_kernel(const SweepArgs A) {
     const int nact = *A.nlist;
     const int g0 = blockIdx.y * W;
     if (g0 >= nact) return;
     int BA, BB, BC;
     decode_brick(A.diag, blockIdx.x, A.nba, A.nbb, A.nbc, &BA, &BB, &BC);
-    brick<GRAD, W>(A, A.pass, BA, BB, BC, g0, nact);
+    brick<GRAD, LOG, W>(A, A.pass, BA, BB, BC, g0, nact);
+}
+
+// Logged solve: closes the current pass's segment of the history.
+__global__ void log_mark_kernel(HistHeader *hdr, unsigned long long *seg,
+                                const unsigned long long *count) {
+    const int P = hdr->npass;
+    if (P < hdr->max_pass) seg[P + 1] = *count;
+    hdr->npass = P + 1;
 }
 
 // Persistent schedule: one launch per sweep set.  Task t (taken in increasing order) is
@@ -539,7 +492,7 @@ __global__ __launch_bounds__(64 * W) void sweep3d_persistent_kernel(const SweepA
         }
         __syncthreads();
         __threadfence();  // acquire for every wave: see the producers' data
-        brick<GRAD, W>(A, p, BA, BB, BC, g * W, nact);
+        brick<GRAD, false, W>(A, p, BA, BB, BC, g * W, nact);
         __threadfence();  // release this thread's writes
         __syncthreads();
         if (tid == 0)
@@ -704,7 +657,12 @@ size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 template <bool G, int W>
 void launch_diag_t(dim3 grid, hipStream_t st, const SweepArgs &a) {
-    hipLaunchKernelGGL((sweep3d_diag_kernel<G, W>), grid, dim3(64 * W), 0, st, a);
+    hipLaunchKernelGGL((sweep3d_diag_kernel<G, false, W>), grid, dim3(64 * W), 0, st, a);
+}
+
+template <bool G, int W>
+void launch_diag_log_t(dim3 grid, hipStream_t st, const SweepArgs &a) {
+    hipLaunchKernelGGL((sweep3d_diag_kernel<false, true, W>), grid, dim3(64 * W), 0, st, a);
 }
 
 template <bool G, int W>
@@ -809,11 +767,26 @@ extern "C" size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc) {
     return ws_bytes(nz, nx, ny, nsrc, nullptr, nullptr);
 }
 
-extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double dz, double dx,
-                             double dy, const double *src, int nsrc, int nsweep, int max_sets,
-                             int want_grad, int tile_a, int group, int schedule, double *tt,
-                             uint8_t *sgn, double *vzero, int *sets_done, void *workspace,
-                             size_t workspace_bytes, int use_graph, void *stream_) {
+namespace {
+
+// Logged solve: initialises the history header in device memory.
+__global__ void log_init_kernel(HistHeader *hdr, unsigned long long *seg, int max_pass,
+                                long long cap, long long nodes, int nsrc) {
+    hdr->count = 0;
+    hdr->npass = 0;
+    hdr->max_pass = max_pass;
+    hdr->cap = cap;
+    hdr->nodes = nodes;
+    hdr->nsrc = nsrc;
+    hdr->pad = 0;
+    seg[0] = 0;
+}
+
+int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double dx, double dy,
+                 const double *src, int nsrc, int nsweep, int max_sets, int want_grad, int tile_a,
+                 int group, int schedule, double *tt, uint8_t *sgn, double *vzero, int *sets_done,
+                 void *workspace, size_t workspace_bytes, int use_graph, void *stream_,
+                 const HistView *hv, long long hcap, long long *events) {
     if (nz < 1 || nx < 1 || ny < 1 || nsrc < 1 || nsrc > 65535 || !slow || !src || !tt ||
         !vzero || !sets_done || !workspace)
         return FTEIK_ERR_ARG;
@@ -866,6 +839,10 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
     a.ndiag = a.nba + a.nbb + a.nbc - 2;
     a.pass = 0;
     a.diag = 0;
+    a.log_node = hv ? hv->node : nullptr;
+    a.log_old = hv ? hv->old : nullptr;
+    a.log_count = hv ? &hv->hdr->count : nullptr;
+    a.log_cap = hv ? hcap : 0;
     a.dz = dz; a.dx = dx; a.dy = dy;
     a.dz2i = 1.0 / dz / dz;
     a.dx2i = 1.0 / dx / dx;
@@ -913,6 +890,12 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
         hipLaunchKernelGGL(zero_bytes_kernel, dim3(1), dim3(64), 0, stream, (uint8_t *)w.error,
                            (long long)sizeof(int));
         FTEIK_CHECK(hipGetLastError());
+        if (hv) {
+            const int max_pass = 8 * (nsweep > 0 ? nsweep : max_sets);
+            hipLaunchKernelGGL(log_init_kernel, dim3(1), dim3(1), 0, stream, hv->hdr, hv->seg,
+                               max_pass, hcap, nn, nsrc);
+            FTEIK_CHECK(hipGetLastError());
+        }
     }
 
     auto enqueue_set = [&]() -> int {
@@ -927,9 +910,17 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
                 a.pass = p;
                 for (int d = 0; d < a.ndiag; ++d) {
                     a.diag = d;
-                    FTEIK_DISPATCH_W(launch_diag_t, want_grad != 0, group,
-                                     dim3(nb[d], (unsigned)gmax), stream, a);
+                    if (hv) {
+                        FTEIK_DISPATCH_W(launch_diag_log_t, false, group,
+                                         dim3(nb[d], (unsigned)gmax), stream, a);
+                    } else {
+                        FTEIK_DISPATCH_W(launch_diag_t, want_grad != 0, group,
+                                         dim3(nb[d], (unsigned)gmax), stream, a);
+                    }
                 }
+                if (hv)
+                    hipLaunchKernelGGL(log_mark_kernel, dim3(1), dim3(1), 0, stream, hv->hdr,
+                                       hv->seg, (const unsigned long long *)&hv->hdr->count);
             }
         }
         hipLaunchKernelGGL(set_bookkeeping_kernel, dim3(1), dim3(256), 0, stream, nsrc, sets_done,
@@ -975,7 +966,52 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
         status = hip_status(e);
         if (!status && err) status = FTEIK_ERR_TIMEOUT;
     }
+    if (!status && hv) {
+        unsigned long long count = 0;
+        hipError_t e = hipMemcpyAsync(&count, &hv->hdr->count, sizeof(count),
+                                      hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        status = hip_status(e);
+        if (events) *events = (long long)count;
+        if (!status && count > (unsigned long long)hcap) status = FTEIK_ERR_WORKSPACE;
+    }
     return status;
+}
+
+}  // namespace
+
+extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double dz, double dx,
+                             double dy, const double *src, int nsrc, int nsweep, int max_sets,
+                             int want_grad, int tile_a, int group, int schedule, double *tt,
+                             uint8_t *sgn, double *vzero, int *sets_done, void *workspace,
+                             size_t workspace_bytes, int use_graph, void *stream) {
+    return solve3d_impl(slow, nz, nx, ny, dz, dx, dy, src, nsrc, nsweep, max_sets, want_grad,
+                        tile_a, group, schedule, tt, sgn, vzero, sets_done, workspace,
+                        workspace_bytes, use_graph, stream, nullptr, 0, nullptr);
+}
+
+extern "C" size_t fteik3d_history_size(long long events, int max_passes) {
+    if (events < 0 || max_passes < 1) return 0;
+    return hist_bytes(events, max_passes);
+}
+
+extern "C" int fteik3d_solve_logged(const double *slow, int nz, int nx, int ny, double dz,
+                                    double dx, double dy, const double *src, int nsrc, int nsweep,
+                                    int max_sets, int tile_a, int group, double *tt, double *vzero,
+                                    int *sets_done, void *workspace, size_t workspace_bytes,
+                                    void *history, size_t history_bytes, long long *events,
+                                    int use_graph, void *stream) {
+    if (!history || nz < 1 || nx < 1 || ny < 1 || nsrc < 1) return FTEIK_ERR_ARG;
+    if (nsweep <= 0 && max_sets < 1) return FTEIK_ERR_ARG;
+    // logged node indices are 32-bit
+    if ((long long)(nz + 1) * (nx + 1) * (ny + 1) * nsrc > 0xffffffffLL) return FTEIK_ERR_ARG;
+    const int max_pass = 8 * (nsweep > 0 ? nsweep : max_sets);
+    const long long cap = hist_capacity(history_bytes, max_pass);
+    if (cap < 1) return FTEIK_ERR_WORKSPACE;
+    const HistView hv = hist_view(history, cap, max_pass);
+    return solve3d_impl(slow, nz, nx, ny, dz, dx, dy, src, nsrc, nsweep, max_sets, 0, tile_a,
+                        group, 1, tt, nullptr, vzero, sets_done, workspace, workspace_bytes,
+                        use_graph, stream, &hv, cap, events);
 }
 
 extern "C" int fteik3d_gradient(const double *tt, const uint8_t *sgn, int nz, int nx, int ny,

@@ -33,6 +33,7 @@ extern "C" {
 #define FTEIK_ERR_WORKSPACE 5
 #define FTEIK_ERR_TIMEOUT 6
 #define FTEIK_ERR_UNSUPPORTED 7
+#define FTEIK_ERR_UNSETTLED 8
 
 /* Library version string. */
 const char *fteik_version(void);
@@ -85,6 +86,86 @@ int fteik3d_solve(const double *slow, int nz, int nx, int ny, double dz, double 
 int fteik3d_gradient(const double *tt, const uint8_t *sgn, int nz, int nx, int ny, double dz,
                      double dx, double dy, const double *src, double vzero, double *ttgrad,
                      void *stream);
+
+/* ---------------------------------------------------------------------------------- */
+/* 3D adjoint-state gradient                                                           */
+/* ---------------------------------------------------------------------------------- */
+
+/* Bytes of a sweep-history buffer holding `events` node changes over at most max_passes
+ * passes (8 per sweep set). */
+size_t fteik3d_history_size(long long events, int max_passes);
+
+/*
+ * No counterpart in the reference: fteik3d_solve (exact schedule, no gradient signs) that
+ * also records its sweep history -- every node change (node, time before the change),
+ * pass by pass -- into `history` for fteik3d_adjoint_history.  Arguments as
+ * fteik3d_solve; `history` is device memory of history_bytes (fteik3d_history_size with
+ * max_passes = 8 * nsweep, or 8 * max_sets when converging).  *events receives the number
+ * of changes; FTEIK_ERR_WORKSPACE means the buffer was too small for them (tt is still
+ * correct; retry with fteik3d_history_size(*events, ...)).  nsrc * (nz+1)(nx+1)(ny+1) must
+ * be below 2^32.
+ */
+int fteik3d_solve_logged(const double *slow, int nz, int nx, int ny, double dz, double dx,
+                         double dy, const double *src, int nsrc, int nsweep, int max_sets,
+                         int tile_a, int group, double *tt, double *vzero, int *sets_done,
+                         void *workspace, size_t workspace_bytes, void *history,
+                         size_t history_bytes, long long *events, int use_graph, void *stream);
+
+/* Device workspace (bytes) of fteik3d_adjoint_history for a history of `events` changes. */
+size_t fteik3d_adjoint_history_workspace_size(int nz, int nx, int ny, int nsrc,
+                                              long long events);
+
+/*
+ * No counterpart in the reference: the exact gradient of a receiver-traveltime loss L with
+ * respect to slowness -- reverse mode through the sweep history of fteik3d_solve_logged --
+ * for T_rec[s, r] = vinterp3d of the solved grids (TraveltimeGrid3D.__call__,
+ * fteikpy/_grid.py:342-368, on Eikonal3D.solve grids, fteikpy/_solver.py:120-126).
+ * Passes are reversed last to first; the winning candidate of every logged change is
+ * recomputed on the grid as it stood after its pass.
+ *
+ * tt       device [nsrc, nz+1, nx+1, ny+1]: the grids the logged solve returned; they are
+ *          rolled back in place to the initial state (the caller keeps a copy if needed)
+ * history  the logged solve's history buffer (read only)
+ * origin, src, rec, dldt, dlds as fteik3d_adjoint (src absolute, host; dlds overwritten)
+ * Returns 0, FTEIK_ERR_ARG (history of another grid / source count),
+ * FTEIK_ERR_WORKSPACE (workspace too small or incomplete history), FTEIK_ERR_SOURCE_OOB,
+ * FTEIK_ERR_UNSETTLED (a logged change is not reproduced by its candidate) or a negative
+ * HIP error.  Synchronizes the stream once per pass that changed nodes.
+ */
+int fteik3d_adjoint_history(const double *slow, int nz, int nx, int ny, double dz, double dx,
+                            double dy, const double *origin, const double *src, int nsrc,
+                            double *tt, const void *history, const double *rec, int nrec,
+                            const double *dldt, double *dlds, void *workspace,
+                            size_t workspace_bytes, void *stream);
+
+/* Device workspace (bytes) fteik3d_adjoint needs: 21 bytes per node and source. */
+size_t fteik3d_adjoint_workspace_size(int nz, int nx, int ny, int nsrc);
+
+/*
+ * No counterpart in the reference: the gradient of a receiver-traveltime loss L with
+ * respect to slowness, for T_rec[s, r] = vinterp3d of converged fteik3d grids (what
+ * TraveltimeGrid3D.__call__ returns, fteikpy/_grid.py:342-368, on the grids of
+ * Eikonal3D.solve, fteikpy/_solver.py:120-126).  The upwind stencil of every node is
+ * recomputed from the converged times and the adjoint system is solved exactly in reverse
+ * dependency order.
+ *
+ * slow    device [nz, nx, ny] slowness the grids were solved with
+ * origin  host [3] grid origin; src: host [nsrc, 3] ABSOLUTE source coordinates
+ * tt      device [nsrc, nz+1, nx+1, ny+1] converged traveltimes (fteik3d_solve, nsweep <= 0)
+ * rec     device [nrec, 3] absolute receiver coordinates (shared by all sources)
+ * dldt    device [nsrc, nrec] dL/dT_rec
+ * dlds    device [nz, nx, ny] output dL/dslow (overwritten)
+ * stale   host out: nodes carrying adjoint whose held time no current candidate reproduces
+ *         (0 when the sweeps reached a consistent fixed point); may be NULL
+ * rounds  host out: dependency rounds executed; may be NULL
+ * Returns 0, FTEIK_ERR_SOURCE_OOB, FTEIK_ERR_ARG (also for nsrc * nodes >= 2^32),
+ * FTEIK_ERR_WORKSPACE, FTEIK_ERR_UNSETTLED (the stencil graph has a cycle) or a negative
+ * HIP error.  Synchronizes the stream once per dependency round.
+ */
+int fteik3d_adjoint(const double *slow, int nz, int nx, int ny, double dz, double dx, double dy,
+                    const double *origin, const double *src, int nsrc, const double *tt,
+                    const double *rec, int nrec, const double *dldt, double *dlds, int *stale,
+                    int *rounds, void *workspace, size_t workspace_bytes, void *stream);
 
 /* ---------------------------------------------------------------------------------- */
 /* 2D solve                                                                            */

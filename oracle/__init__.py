@@ -12,6 +12,9 @@ numba entry points:
 - ``interp3d`` / ``interp2d``   -> fteikpy/_interp/_interp3d.py:188-195, _interp2d.py:88-95
 - ``vinterp3d`` / ``vinterp2d`` -> fteikpy/_interp/_vinterp3d.py:280-291, _vinterp2d.py:122-131
 - ``ray3d`` / ``ray2d``         -> fteikpy/_fteik/_ray3d.py:167-208, _ray2d.py:120-153
+- ``adjoint3d``  -> no reference counterpart: adjoint-state gradient of receiver traveltimes
+  (vinterp3d of converged fteik3d grids) with respect to slowness, pinned by central finite
+  differences of ``solve3d`` + ``vinterp3d``
 
 Extra keyword arguments (not in the reference): ``nsweep <= 0`` runs sets until one
 changes no node (``max_sets`` caps it) and the number of sets executed is returned;
@@ -53,7 +56,9 @@ def lib():
         L.oracle_vinterp2d.argtypes = [_dp, i, _dp, i, _dp, _dp, i, _dp, d, d, _dp]
         L.oracle_ray3d.argtypes = [_dp, i, _dp, i, _dp, i, _dp, _dp, _dp, _dp, i, _dp, d, i, i, _dp, _ip, _ip]
         L.oracle_ray2d.argtypes = [_dp, i, _dp, i, _dp, _dp, _dp, i, _dp, d, i, i, _dp, _ip, _ip]
-        for name in ("oracle_fteik3d_batch", "oracle_fteik2d_batch", "oracle_ray3d", "oracle_ray2d"):
+        L.oracle_adjoint3d.argtypes = [_dp, i, i, i, d, d, d, _dp, _dp, i, _dp, _dp, i, _dp, _dp, _ip, _ip]
+        L.oracle_adjoint3d_exact.argtypes = [_dp, i, i, i, d, d, d, _dp, _dp, i, i, _dp, i, _dp, _dp, _ip]
+        for name in ("oracle_adjoint3d_exact", "oracle_adjoint3d", "oracle_fteik3d_batch", "oracle_fteik2d_batch", "oracle_ray3d", "oracle_ray2d"):
             getattr(L, name).restype = ctypes.c_int
     return _lib
 
@@ -190,3 +195,52 @@ def ray2d(z, x, gz, gx, p, src, stepsize, max_step, honor_grid=False):
     _ray_errors(status)
     out = [rays[r, counts[r]::-1].copy() for r in range(n)]
     return out[0] if single else out
+
+
+def adjoint3d(slow, dz, dx, dy, origin, src, tt, rec, dldt):
+    """dL/d(slow) given dL/dT_rec for T_rec = vinterp3d of the converged grids ``tt``.
+
+    ``src`` (nsrc, 3) and ``rec`` (nrec, 3) are absolute, ``tt`` is (nsrc, nz+1, nx+1, ny+1),
+    ``dldt`` is (nsrc, nrec).  Returns (dlds, orphans, rounds): the number of stale nodes
+    (held values no candidate reproduces) that carried adjoint, and the push rounds needed.
+    """
+    slow = _f8(slow)
+    src = _f8(src).reshape(-1, 3)
+    rec = _f8(rec).reshape(-1, 3)
+    tt = _f8(tt).reshape((len(src),) + tuple(s + 1 for s in slow.shape))
+    dldt = _f8(dldt).reshape(len(src), len(rec))
+    nz, nx, ny = slow.shape
+    dlds = np.empty_like(slow)
+    orph = np.zeros(1, dtype=np.int32)
+    rounds = np.zeros(1, dtype=np.int32)
+    st = lib().oracle_adjoint3d(
+        _p(slow), nz, nx, ny, dz, dx, dy, _p(_f8(origin)), _p(src), len(src), _p(tt), _p(rec),
+        len(rec), _p(dldt), _p(dlds), _pi(orph), _pi(rounds))
+    if st == 1:
+        raise ValueError("source out of bound")
+    if st == 2:
+        raise RuntimeError("adjoint pushes did not settle")
+    return dlds, int(orph[0]), int(rounds[0])
+
+
+def adjoint3d_exact(slow, dz, dx, dy, origin, src, rec, dldt, max_sets=100):
+    """Exact discrete adjoint: reverse mode through the converged solve's sweep history.
+
+    Solves internally (converge mode, at most ``max_sets`` sets).  Returns (dlds, sets) with
+    sets the longest history in sets.
+    """
+    slow = _f8(slow)
+    src = _f8(src).reshape(-1, 3)
+    rec = _f8(rec).reshape(-1, 3)
+    dldt = _f8(dldt).reshape(len(src), len(rec))
+    nz, nx, ny = slow.shape
+    dlds = np.empty_like(slow)
+    sets = np.zeros(1, dtype=np.int32)
+    st = lib().oracle_adjoint3d_exact(
+        _p(slow), nz, nx, ny, dz, dx, dy, _p(_f8(origin)), _p(src), len(src), int(max_sets),
+        _p(rec), len(rec), _p(dldt), _p(dlds), _pi(sets))
+    if st == 1:
+        raise ValueError("source out of bound")
+    if st == 2:
+        raise RuntimeError("history replay mismatch")
+    return dlds, int(sets[0])

@@ -29,6 +29,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <stdio.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -79,11 +80,12 @@ typedef struct {
     double dz, dx, dy, dz2i, dx2i, dy2i, dz2dx2, dz2dy2, dx2dy2, dsum;
 } dargs3_t;
 
-/* Node update, _fteik3d.py:38-200.  nz, nx, ny are NODE counts here as in the reference. */
-static inline void sweep_node3(double *tt, int32_t *sgn, const double *slow, const dargs3_t *d,
-                               int i, int j, int k, int sgnvz, int sgnvx, int sgnvy,
-                               int sgntz, int sgntx, int sgnty, int nz, int nx, int ny,
-                               int grad) {
+/* Candidates of the node update, _fteik3d.py:38-164: returns min(t0, t1d, t2d, t3d) and,
+ * in *which, the winning candidate in the reference's priority order (:166-200) as a set of
+ * axes (bit 0 z, bit 1 x, bit 2 y).  nz, nx, ny are NODE counts here as in the reference. */
+static inline double cand3(const double *tt, const double *slow, const dargs3_t *d, int i,
+                           int j, int k, int sgnvz, int sgnvx, int sgnvy, int sgntz, int sgntx,
+                           int sgnty, int nz, int nx, int ny, double t0, int *which) {
 #define TT(a, b, c) tt[((size_t)(a) * nx + (b)) * ny + (c)]
 #define SL(a, b, c) slow[((size_t)(a) * (nx - 1) + (b)) * (ny - 1) + (c)]
     const double dz = d->dz, dx = d->dx, dy = d->dy;
@@ -163,21 +165,36 @@ static inline void sweep_node3(double *tt, int32_t *sgn, const double *slow, con
     }
 
     /* :162-200 */
-    double t0 = TT(i, j, k);
     double tnew = pmin2(pmin2(pmin2(t0, t1d), t2d), t3d);
-    TT(i, j, k) = tnew;
-    if (grad && tnew != t0) {
-        int32_t *s = sgn + (((size_t)i * nx + j) * ny + k) * 3;
-        if (tnew == t1d1) { s[0] = sgntz; s[1] = 0; s[2] = 0; }
-        else if (tnew == t1d2) { s[0] = 0; s[1] = sgntx; s[2] = 0; }
-        else if (tnew == t1d3) { s[0] = 0; s[1] = 0; s[2] = sgnty; }
-        else if (tnew == t2d1) { s[0] = sgntz; s[1] = sgntx; s[2] = 0; }
-        else if (tnew == t2d2) { s[0] = sgntz; s[1] = 0; s[2] = sgnty; }
-        else if (tnew == t2d3) { s[0] = 0; s[1] = sgntx; s[2] = sgnty; }
-        else { s[0] = sgntz; s[1] = sgntx; s[2] = sgnty; }
-    }
+    if (tnew == t1d1) *which = 1;
+    else if (tnew == t1d2) *which = 2;
+    else if (tnew == t1d3) *which = 4;
+    else if (tnew == t2d1) *which = 3;
+    else if (tnew == t2d2) *which = 5;
+    else if (tnew == t2d3) *which = 6;
+    else *which = 7;
+    return tnew;
 #undef TT
 #undef SL
+}
+
+/* Node update, _fteik3d.py:38-200. */
+static inline void sweep_node3(double *tt, int32_t *sgn, const double *slow, const dargs3_t *d,
+                               int i, int j, int k, int sgnvz, int sgnvx, int sgnvy,
+                               int sgntz, int sgntx, int sgnty, int nz, int nx, int ny,
+                               int grad) {
+    size_t n = ((size_t)i * nx + j) * ny + k;
+    double t0 = tt[n];
+    int which;
+    double tnew = cand3(tt, slow, d, i, j, k, sgnvz, sgnvx, sgnvy, sgntz, sgntx, sgnty, nz, nx,
+                        ny, t0, &which);
+    tt[n] = tnew;
+    if (grad && tnew != t0) {
+        int32_t *s = sgn + n * 3;
+        s[0] = (which & 1) ? sgntz : 0;
+        s[1] = (which & 2) ? sgntx : 0;
+        s[2] = (which & 4) ? sgnty : 0;
+    }
 }
 
 /* One sweeping set, _fteik3d.py:203-391.  `order` 0 = the reference's k,j,i nesting,
@@ -1129,4 +1146,491 @@ int oracle_ray2d(const double *z, int nz, const double *x, int nx, const double 
         if (st && !first) first = st;
     }
     return first;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Adjoint-state gradient of receiver traveltimes (no counterpart in the reference).     */
+/* ------------------------------------------------------------------------------------ */
+/*
+ * Differentiates the converged fixed point of fteik3d and the receiver interpolation of
+ * vinterp3d.  At a converged grid every lowered node equals one candidate of the node
+ * update (cand3) for one sweep direction, evaluated on the final neighbour times; the
+ * selection is recomputed here by evaluating all 8 directions and taking the first whose
+ * minimum equals the node's time.  The adjoint of the triangular system is accumulated in
+ * decreasing-traveltime order (every upwind neighbour of a selected stencil is earlier),
+ * which is checked for every push.  Pinned by central finite differences of
+ * oracle_fteik3d + vinterp3_one (tests/test_oracle_adjoint.py).
+ */
+
+/* Partial derivatives of the selected candidate: upwind node offsets by subset mask b of
+ * the axes (bit 0 z, bit 1 x, bit 2 y), dT/dT_up[b], the slowness cell used and dT/ds. */
+typedef struct {
+    double w[8];
+    size_t up[8];
+    int has[8];
+    size_t cell;
+    double wcell;
+} adj_stencil_t;
+
+static void adj_stencil3(const double *tt, const double *slow, const dargs3_t *d, int i, int j,
+                         int k, int sz, int sx, int sy, int which, int NZ, int NX, int NY,
+                         adj_stencil_t *S) {
+#define NODE(a, b, c) (((size_t)(a) * NX + (b)) * NY + (c))
+#define CELL(a, b, c) (((size_t)(a) * (NX - 1) + (b)) * (NY - 1) + (c))
+    for (int b = 0; b < 8; ++b) {
+        S->w[b] = 0.0;
+        S->has[b] = 0;
+        S->up[b] = 0;
+    }
+    int dz_ = (which & 1) ? sz : 0, dx_ = (which & 2) ? sx : 0, dy_ = (which & 4) ? sy : 0;
+    for (int b = 1; b < 8; ++b)
+        if ((b & which) == b) {
+            S->has[b] = 1;
+            S->up[b] = NODE(i - ((b & 1) ? dz_ : 0), j - ((b & 2) ? dx_ : 0), k - ((b & 4) ? dy_ : 0));
+        }
+    int i1 = i - (sz > 0), j1 = j - (sx > 0), k1 = k - (sy > 0);
+    int jm = imax(j - 1, 0), jp = imin(j, NX - 2);
+    int km = imax(k - 1, 0), kp = imin(k, NY - 2);
+    int im = imax(i - 1, 0), ip = imin(i, NZ - 2);
+    /* the cell achieving a min(): first of equal values, like Python's min() */
+    size_t c4[4];
+    int nc = 0;
+    double dd = 0.0;
+    if (which == 1 || which == 2 || which == 4) {
+        if (which == 1) {
+            c4[0] = CELL(i1, jm, km); c4[1] = CELL(i1, jm, kp); c4[2] = CELL(i1, jp, km); c4[3] = CELL(i1, jp, kp);
+            dd = d->dz;
+        } else if (which == 2) {
+            c4[0] = CELL(im, j1, km); c4[1] = CELL(ip, j1, km); c4[2] = CELL(im, j1, kp); c4[3] = CELL(ip, j1, kp);
+            dd = d->dx;
+        } else {
+            c4[0] = CELL(im, jm, k1); c4[1] = CELL(im, jp, k1); c4[2] = CELL(ip, jm, k1); c4[3] = CELL(ip, jp, k1);
+            dd = d->dy;
+        }
+        nc = 4;
+    } else if (which != 7) {
+        if (which == 3) { c4[0] = CELL(i1, j1, km); c4[1] = CELL(i1, j1, kp); }
+        else if (which == 5) { c4[0] = CELL(i1, jm, k1); c4[1] = CELL(i1, jp, k1); }
+        else { c4[0] = CELL(im, j1, k1); c4[1] = CELL(ip, j1, k1); }
+        nc = 2;
+    } else {
+        c4[0] = CELL(i1, j1, k1);
+        nc = 1;
+    }
+    size_t best = c4[0];
+    for (int c = 1; c < nc; ++c)
+        if (slow[c4[c]] < slow[best]) best = c4[c];
+    double v = slow[best];
+    S->cell = best;
+    if (nc == 4) {
+        /* T = T_up + d * v */
+        S->w[which] = 1.0;
+        S->wcell = dd;
+    } else if (nc == 2) {
+        /* Plane-wave operator with axes (a, b) in z, x, y order, a1 = upwind time along a,
+         * b1 = along b, ab = diagonal.  All three planes read
+         *   T = (a2i (ab + a1 - b1) + b2i (ab + b1 - a1) + sqrt(R)) / (a2i + b2i),
+         *   R = 4 v^2 (a2i + b2i) - a2i b2i (2 (b1 - a1))^2. */
+        int ba = which & 1 ? 1 : 2, bb = which & 4 ? 4 : 2;
+        double a2i = ba == 1 ? d->dz2i : d->dx2i, b2i = bb == 4 ? d->dy2i : d->dx2i;
+        double a1 = tt[S->up[ba]], b1 = tt[S->up[bb]];
+        double q = 2.0 * (b1 - a1);
+        double R = 4.0 * v * v * (a2i + b2i) - a2i * b2i * q * q;
+        double sR = sqrt(R), sum = a2i + b2i;
+        /* dR/da1 = 4 a2i b2i q, dR/db1 = -4 a2i b2i q */
+        S->w[ba] = (a2i - b2i + 2.0 * a2i * b2i * q / sR) / sum;
+        S->w[bb] = (b2i - a2i - 2.0 * a2i * b2i * q / sR) / sum;
+        S->w[ba | bb] = 1.0;
+        S->wcell = 4.0 * v / sR;
+    } else {
+        /* 3D operator: T = (tb z2i + ta x2i + tc y2i + sqrt(9 v^2 dsum - Q)) / dsum with
+         * ta, tb, tc linear in the 7 upwind times (_fteik3d.py:150-152). */
+        double tv = tt[S->up[1]], te = tt[S->up[2]], tn = tt[S->up[4]];
+        double tev = tt[S->up[3]], tnv = tt[S->up[5]], ten = tt[S->up[6]], tnve = tt[S->up[7]];
+        double ta = te - 0.5 * tn + 0.5 * ten - 0.5 * tv + 0.5 * tev - tnv + tnve;
+        double tb = tv - 0.5 * tn + 0.5 * tnv - 0.5 * te + 0.5 * tev - ten + tnve;
+        double tc = tn - 0.5 * te + 0.5 * ten - 0.5 * tv + 0.5 * tnv - tev + tnve;
+        double Q = d->dz2dx2 * (ta - tb) * (ta - tb) + d->dz2dy2 * (tb - tc) * (tb - tc) +
+                   d->dx2dy2 * (ta - tc) * (ta - tc);
+        double sR = sqrt(9.0 * v * v * d->dsum - Q);
+        /* dT/dta etc. */
+        double ga = (d->dx2i - (d->dz2dx2 * (ta - tb) + d->dx2dy2 * (ta - tc)) / sR) / d->dsum;
+        double gb = (d->dz2i - (-d->dz2dx2 * (ta - tb) + d->dz2dy2 * (tb - tc)) / sR) / d->dsum;
+        double gc = (d->dy2i - (-d->dz2dy2 * (tb - tc) - d->dx2dy2 * (ta - tc)) / sR) / d->dsum;
+        /* coefficients of (tv, te, tn, tev, tnv, ten, tnve) in (ta, tb, tc) */
+        S->w[1] = -0.5 * ga + gb - 0.5 * gc;
+        S->w[2] = ga - 0.5 * gb - 0.5 * gc;
+        S->w[4] = -0.5 * ga - 0.5 * gb + gc;
+        S->w[3] = 0.5 * ga + 0.5 * gb - gc;
+        S->w[5] = -ga + 0.5 * gb + 0.5 * gc;
+        S->w[6] = 0.5 * ga - gb + 0.5 * gc;
+        S->w[7] = ga + gb + gc;
+        S->wcell = 9.0 * v / sR;
+    }
+#undef NODE
+#undef CELL
+}
+
+/* Recomputed selection: the direction (pass order of sweep3d) and candidate whose value is
+ * smallest over the 8 directions, the first one on ties.  Returns the candidate's axis set,
+ * 0 if the node keeps its initial value (no candidate is below BIG), and sets *stale when
+ * that smallest candidate is above the node's time: the node then holds a value computed
+ * from earlier neighbour times (a 2D/3D candidate whose causality condition no longer
+ * holds), and its current best candidate stands in for it. */
+static int adj_select3(const double *tt, const double *slow, const dargs3_t *d, int i, int j,
+                       int k, int NZ, int NX, int NY, int *sz, int *sx, int *sy, int *stale) {
+    static const int dirs[8][3] = {{1, 1, 1},  {1, -1, 1},  {1, 1, -1},  {1, -1, -1},
+                                   {-1, 1, 1}, {-1, -1, 1}, {-1, 1, -1}, {-1, -1, -1}};
+    double t = tt[((size_t)i * NX + j) * NY + k];
+    double best = BIG;
+    int bw = 0;
+    for (int p = 0; p < 8; ++p) {
+        int a = dirs[p][0], b = dirs[p][1], c = dirs[p][2];
+        if (i - a < 0 || i - a >= NZ || j - b < 0 || j - b >= NX || k - c < 0 || k - c >= NY)
+            continue;
+        int which;
+        double m = cand3(tt, slow, d, i, j, k, a > 0, b > 0, c > 0, a, b, c, NZ, NX, NY, BIG,
+                         &which);
+        if (m < best) {
+            best = m;
+            bw = which;
+            *sz = a;
+            *sx = b;
+            *sy = c;
+        }
+    }
+    if (!(best <= t)) {
+        /* above the node's time: a seeded node keeps its initial value; otherwise stale */
+        *stale = 1;
+    } else {
+        *stale = 0;
+    }
+    return bw;
+}
+
+static const double *g_sort_tt; /* qsort has no context argument */
+static int adj_cmp_desc(const void *a, const void *b) {
+    double ta = g_sort_tt[*(const size_t *)a], tb = g_sort_tt[*(const size_t *)b];
+    return ta < tb ? 1 : (ta > tb ? -1 : 0);
+}
+
+/* dT/d(corner times) of vinterp3_one for one receiver, or the vzero path.  Adds
+ * w * dT/dt_c into lam and w * dT/dvzero into *dvz.  Axes are absolute, v index [z][x][y]
+ * (the vinterp3_one call convention with x <- z axis etc.). */
+static void adj_receiver3(const double *z, int nz, const double *x, int nx, const double *y,
+                          int ny, const double *tt, double zq, double xq, double yq, double zs,
+                          double xs, double ys, double w, double *lam, double *dvz) {
+    if (!(z[0] <= zq && zq <= z[nz - 1] && x[0] <= xq && xq <= x[nx - 1] && y[0] <= yq &&
+          yq <= y[ny - 1]))
+        return;
+    double dist = dist3(zs, xs, ys, zq, xq, yq);
+    int zsi = ssr(z, nz, zs), xsi = ssr(x, nx, xs), ysi = ssr(y, ny, ys);
+    int i1 = ssr(z, nz, zq), j1 = ssr(x, nx, xq), k1 = ssr(y, ny, yq);
+    if (zsi == i1 && xsi == j1 && ysi == k1) {
+        *dvz += w * dist;
+        return;
+    }
+    cell3_t c = cell3(z, nz, x, nx, y, ny, i1, j1, k1);
+    double vol = fabs((c.x2 - c.x1) * (c.y2 - c.y1) * (c.z2 - c.z1));
+    double a[8], t[8];
+    size_t id[8];
+    int ok[8];
+    double V = 0.0;
+    for (int q = 0; q < 8; ++q) {
+        int di = q & 1, dj = (q >> 1) & 1, dk = (q >> 2) & 1;
+        ok[q] = !((di && !c.ok2x) || (dj && !c.ok2y) || (dk && !c.ok2z));
+        a[q] = 0.0;
+        t[q] = 1.0;
+        if (!ok[q]) continue;
+        double pz = di ? c.x2 : c.x1, px = dj ? c.y2 : c.y1, py = dk ? c.z2 : c.z1;
+        double wq = fabs(((di ? c.x1 : c.x2) - zq) * ((dj ? c.y1 : c.y2) - xq) *
+                         ((dk ? c.z1 : c.z2) - yq));
+        id[q] = ((size_t)(i1 + di) * nx + (j1 + dj)) * ny + (k1 + dk);
+        t[q] = tt[id[q]];
+        if (t[q] == 0.0) {
+            *dvz += w * dist;
+            return;
+        }
+        a[q] = dist3(zs, xs, ys, pz, px, py) * wq / vol;
+        V += a[q] / t[q];
+    }
+    for (int q = 0; q < 8; ++q)
+        if (ok[q]) lam[id[q]] += w * dist * a[q] / (V * V * t[q] * t[q]);
+}
+
+/*
+ * d L / d slow for L with dL/dT_rec = dldt[s][r] (T_rec = vinterp3 of source s at receiver
+ * r).  origin: grid origin; src, rec: absolute coordinates; tt: converged grids stacked per
+ * source.  dlds (nz*nx*ny) is overwritten.  *orphans counts the stale nodes (see
+ * adj_select3) with non-zero adjoint; *rounds_out the largest number of push rounds a
+ * source needed (2 when every selected stencil reads only earlier nodes: one round and
+ * the empty one that confirms it).  Returns 0, 1 for a source out of bounds, 2 if the pushes do not settle
+ * within 1000 rounds.
+ */
+int oracle_adjoint3d(const double *slow, int nz, int nx, int ny, double dz, double dx,
+                     double dy, const double *origin, const double *src, int nsrc,
+                     const double *tt_all, const double *rec, int nrec, const double *dldt,
+                     double *dlds, int *orphans, int *rounds_out) {
+    int NZ = nz + 1, NX = nx + 1, NY = ny + 1;
+    size_t nn = (size_t)NZ * NX * NY;
+    dargs3_t d;
+    d.dz = dz; d.dx = dx; d.dy = dy;
+    d.dz2i = 1.0 / dz / dz;
+    d.dx2i = 1.0 / dx / dx;
+    d.dy2i = 1.0 / dy / dy;
+    d.dz2dx2 = d.dz2i * d.dx2i;
+    d.dz2dy2 = d.dz2i * d.dy2i;
+    d.dx2dy2 = d.dx2i * d.dy2i;
+    d.dsum = d.dz2i + d.dx2i + d.dy2i;
+    double *za = (double *)malloc(NZ * sizeof(double));
+    double *xa = (double *)malloc(NX * sizeof(double));
+    double *ya = (double *)malloc(NY * sizeof(double));
+    for (int i = 0; i < NZ; ++i) za[i] = origin[0] + dz * (double)i;
+    for (int i = 0; i < NX; ++i) xa[i] = origin[1] + dx * (double)i;
+    for (int i = 0; i < NY; ++i) ya[i] = origin[2] + dy * (double)i;
+    double *lam = (double *)malloc(nn * sizeof(double));
+    size_t *ord = (size_t *)malloc(nn * sizeof(size_t));
+    double *pushed = (double *)malloc(nn * sizeof(double));
+    unsigned char *kind = (unsigned char *)malloc(nn);
+    adj_stencil_t *st = (adj_stencil_t *)malloc(nn * sizeof(adj_stencil_t));
+    if (rounds_out) *rounds_out = 0;
+    memset(dlds, 0, (size_t)nz * nx * ny * sizeof(double));
+    int status = 0, norph = 0;
+    for (int s = 0; s < nsrc && !status; ++s) {
+        const double *tt = tt_all + nn * s;
+        double zr = src[3 * s] - origin[0], xr = src[3 * s + 1] - origin[1],
+               yr = src[3 * s + 2] - origin[2];
+        if (!(0.0 <= zr && zr <= dz * nz && 0.0 <= xr && xr <= dx * nx && 0.0 <= yr &&
+              yr <= dy * ny)) {
+            status = 1;
+            break;
+        }
+        double zsa = zr / dz, xsa = xr / dx, ysa = yr / dy;
+        if (zsa >= nz) zsa = zsa - EPS;
+        if (xsa >= nx) xsa = xsa - EPS;
+        if (ysa >= ny) ysa = ysa - EPS;
+        int zsi = imin((int)zsa, nz - 1), xsi = imin((int)xsa, nx - 1), ysi = imin((int)ysa, ny - 1);
+        size_t scell = ((size_t)zsi * nx + xsi) * ny + ysi;
+        memset(lam, 0, nn * sizeof(double));
+        double dvz = 0.0;
+        for (int r = 0; r < nrec; ++r)
+            adj_receiver3(za, NZ, xa, NX, ya, NY, tt, rec[3 * r], rec[3 * r + 1], rec[3 * r + 2],
+                          src[3 * s], src[3 * s + 1], src[3 * s + 2],
+                          dldt[(size_t)s * nrec + r], lam, &dvz);
+        /* selection and stencil of every node, once */
+        for (size_t n = 0; n < nn; ++n) {
+            int i = (int)(n / ((size_t)NX * NY)), j = (int)((n / NY) % NX), k = (int)(n % NY);
+            int sz = 0, sx = 0, sy = 0, stale = 0;
+            int which = adj_select3(tt, slow, &d, i, j, k, NZ, NX, NY, &sz, &sx, &sy, &stale);
+            kind[n] = 0;
+            if (stale || !which) {
+                int di = i - zsi, dj = j - xsi, dk = k - ysi;
+                if (di >= 0 && di <= 1 && dj >= 0 && dj <= 1 && dk >= 0 && dk <= 1) {
+                    kind[n] = 2; /* seeded corner: T = vzero * distance */
+                    continue;
+                }
+                if (!which) continue;
+            }
+            adj_stencil3(tt, slow, &d, i, j, k, sz, sx, sy, which, NZ, NX, NY, st + n);
+            int causal = 1;
+            for (int b = 1; b < 8; ++b)
+                if (st[n].has[b] && !(tt[st[n].up[b]] < tt[n])) causal = 0;
+            /* a stale node's stand-in must only read earlier nodes, else it is dropped */
+            if (stale && !causal) continue;
+            kind[n] = stale ? 3 : 1;
+        }
+        for (size_t n = 0; n < nn; ++n) ord[n] = n;
+        g_sort_tt = tt;
+        qsort(ord, nn, sizeof(size_t), adj_cmp_desc);
+        /* lambda = g + J^T lambda: push, in decreasing-time order, the part of each node's
+         * lambda not pushed yet; a candidate reading a later node (the 3D operator is not
+         * strictly causal) is caught up by the next round */
+        memset(pushed, 0, nn * sizeof(double));
+        int round = 0;
+        double scale = 0.0;
+        for (;;) {
+            double dmax = 0.0, lmax = 0.0;
+            for (size_t o = 0; o < nn; ++o) {
+                size_t n = ord[o];
+                double l = lam[n] - pushed[n];
+                if (l == 0.0) continue;
+                pushed[n] = lam[n];
+                if (fabs(l) > dmax) dmax = fabs(l);
+                if (fabs(lam[n]) > lmax) lmax = fabs(lam[n]);
+                if (kind[n] == 2) {
+                    int i = (int)(n / ((size_t)NX * NY)), j = (int)((n / NY) % NX), k = (int)(n % NY);
+                    dvz += l * t_ana3(i, j, k, dz, dx, dy, zsa, xsa, ysa, 1.0);
+                    continue;
+                }
+                if (kind[n] == 0) {
+                    if (round == 0 && tt[n] < BIG) ++norph;
+                    continue;
+                }
+                if (kind[n] == 3 && round == 0) ++norph;
+                const adj_stencil_t *S = st + n;
+                dlds[S->cell] += l * S->wcell;
+                for (int b = 1; b < 8; ++b)
+                    if (S->has[b]) lam[S->up[b]] += l * S->w[b];
+            }
+            if (round == 0) scale = lmax;
+            ++round;
+            /* increments below the rounding of the adjoint itself change nothing */
+            if (dmax <= 1e-16 * scale) break;
+            if (round >= 1000) {
+                status = 2;
+                break;
+            }
+        }
+        if (rounds_out && round > *rounds_out) *rounds_out = round;
+        dlds[scell] += dvz;
+    }
+    free(za);
+    free(xa);
+    free(ya);
+    free(lam);
+    free(ord);
+    free(pushed);
+    free(kind);
+    free(st);
+    if (orphans) *orphans = norph;
+    return status;
+}
+
+/*
+ * Exact discrete adjoint: reverse mode through the sweep history of oracle_fteik3d in
+ * converge mode.  The state before every pass is kept; passes are reversed last to first,
+ * nodes in the reverse of the pass's loop order.  A node that changed in the pass has the
+ * adjoint of its new value pushed through the candidate that produced it, evaluated on the
+ * state after the pass (its upwind neighbours were final for the pass when it was
+ * updated; the winner is recomputed as the candidate equal to the new value), and the
+ * adjoint of its old value is zero; a node that did not change carries its adjoint to the
+ * previous pass.  What is left on the initial state reaches vzero through the 8 seeded
+ * corners.  Same arguments and outputs as oracle_adjoint3d (no stale nodes exist here);
+ * *rounds_out receives the number of sets of the longest history.
+ */
+int oracle_adjoint3d_exact(const double *slow, int nz, int nx, int ny, double dz, double dx,
+                           double dy, const double *origin, const double *src, int nsrc,
+                           int max_sets, const double *rec, int nrec, const double *dldt,
+                           double *dlds, int *rounds_out) {
+    static const int dirs[8][3] = {{1, 1, 1},  {1, -1, 1},  {1, 1, -1},  {1, -1, -1},
+                                   {-1, 1, 1}, {-1, -1, 1}, {-1, 1, -1}, {-1, -1, -1}};
+    int NZ = nz + 1, NX = nx + 1, NY = ny + 1;
+    size_t nn = (size_t)NZ * NX * NY;
+    dargs3_t d;
+    d.dz = dz; d.dx = dx; d.dy = dy;
+    d.dz2i = 1.0 / dz / dz;
+    d.dx2i = 1.0 / dx / dx;
+    d.dy2i = 1.0 / dy / dy;
+    d.dz2dx2 = d.dz2i * d.dx2i;
+    d.dz2dy2 = d.dz2i * d.dy2i;
+    d.dx2dy2 = d.dx2i * d.dy2i;
+    d.dsum = d.dz2i + d.dx2i + d.dy2i;
+    double *za = (double *)malloc(NZ * sizeof(double));
+    double *xa = (double *)malloc(NX * sizeof(double));
+    double *ya = (double *)malloc(NY * sizeof(double));
+    for (int i = 0; i < NZ; ++i) za[i] = origin[0] + dz * (double)i;
+    for (int i = 0; i < NX; ++i) xa[i] = origin[1] + dx * (double)i;
+    for (int i = 0; i < NY; ++i) ya[i] = origin[2] + dy * (double)i;
+    double *lam = (double *)malloc(nn * sizeof(double));
+    double *hist = (double *)malloc(nn * sizeof(double) * (8 * (size_t)max_sets + 1));
+    memset(dlds, 0, (size_t)nz * nx * ny * sizeof(double));
+    if (rounds_out) *rounds_out = 0;
+    int status = 0;
+    for (int s = 0; s < nsrc && !status; ++s) {
+        double zr = src[3 * s] - origin[0], xr = src[3 * s + 1] - origin[1],
+               yr = src[3 * s + 2] - origin[2];
+        if (!(0.0 <= zr && zr <= dz * nz && 0.0 <= xr && xr <= dx * nx && 0.0 <= yr &&
+              yr <= dy * ny)) {
+            status = 1;
+            break;
+        }
+        double zsa = zr / dz, xsa = xr / dx, ysa = yr / dy;
+        if (zsa >= nz) zsa = zsa - EPS;
+        if (xsa >= nx) xsa = xsa - EPS;
+        if (ysa >= ny) ysa = ysa - EPS;
+        int zsi = imin((int)zsa, nz - 1), xsi = imin((int)xsa, nx - 1), ysi = imin((int)ysa, ny - 1);
+        size_t scell = ((size_t)zsi * nx + xsi) * ny + ysi;
+        double vzero = slow[scell];
+        /* forward, keeping the state before every pass (fteik3d, :429-462) */
+        double *t0 = hist;
+        for (size_t n = 0; n < nn; ++n) t0[n] = BIG;
+        static const int corners[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                                          {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+        for (int c = 0; c < 8; ++c) {
+            int i = zsi + corners[c][0], j = xsi + corners[c][1], k = ysi + corners[c][2];
+            t0[((size_t)i * NX + j) * NY + k] = t_ana3(i, j, k, dz, dx, dy, zsa, xsa, ysa, vzero);
+        }
+        int npass = 0, sets = 0;
+        for (int set = 0; set < max_sets; ++set) {
+            int changed = 0;
+            for (int p = 0; p < 8; ++p) {
+                double *cur = hist + nn * (size_t)(npass + 1);
+                memcpy(cur, hist + nn * (size_t)npass, nn * sizeof(double));
+                int sz = dirs[p][0], sx = dirs[p][1], sy = dirs[p][2];
+                int i0 = sz > 0 ? 1 : NZ - 2, ie = sz > 0 ? NZ : -1;
+                int j0 = sx > 0 ? 1 : NX - 2, je = sx > 0 ? NX : -1;
+                int k0 = sy > 0 ? 1 : NY - 2, ke = sy > 0 ? NY : -1;
+                for (int k = k0; k != ke; k += sy)
+                    for (int j = j0; j != je; j += sx)
+                        for (int i = i0; i != ie; i += sz)
+                            sweep_node3(cur, NULL, slow, &d, i, j, k, sz > 0, sx > 0, sy > 0, sz,
+                                        sx, sy, NZ, NX, NY, 0);
+                if (memcmp(cur, hist + nn * (size_t)npass, nn * sizeof(double))) changed = 1;
+                ++npass;
+            }
+            ++sets;
+            if (!changed) break;
+        }
+        if (rounds_out && sets > *rounds_out) *rounds_out = sets;
+        const double *tt = hist + nn * (size_t)npass;
+        /* receiver adjoint on the final grid */
+        memset(lam, 0, nn * sizeof(double));
+        double dvz = 0.0;
+        for (int r = 0; r < nrec; ++r)
+            adj_receiver3(za, NZ, xa, NX, ya, NY, tt, rec[3 * r], rec[3 * r + 1], rec[3 * r + 2],
+                          src[3 * s], src[3 * s + 1], src[3 * s + 2],
+                          dldt[(size_t)s * nrec + r], lam, &dvz);
+        /* reverse the passes */
+        for (int P = npass - 1; P >= 0; --P) {
+            const double *before = hist + nn * (size_t)P, *after = hist + nn * (size_t)(P + 1);
+            int p = P % 8;
+            int sz = dirs[p][0], sx = dirs[p][1], sy = dirs[p][2];
+            int i0 = sz > 0 ? 1 : NZ - 2, ie = sz > 0 ? NZ : -1;
+            int j0 = sx > 0 ? 1 : NX - 2, je = sx > 0 ? NX : -1;
+            int k0 = sy > 0 ? 1 : NY - 2, ke = sy > 0 ? NY : -1;
+            /* reverse of `for k: for j: for i` */
+            for (int k = ke - sy; k != k0 - sy; k -= sy)
+                for (int j = je - sx; j != j0 - sx; j -= sx)
+                    for (int i = ie - sz; i != i0 - sz; i -= sz) {
+                        size_t n = ((size_t)i * NX + j) * NY + k;
+                        if (after[n] == before[n]) continue;
+                        double l = lam[n];
+                        lam[n] = 0.0;
+                        if (l == 0.0) continue;
+                        int which;
+                        double tn = cand3(after, slow, &d, i, j, k, sz > 0, sx > 0, sy > 0, sz, sx,
+                                          sy, NZ, NX, NY, before[n], &which);
+                        if (tn != after[n]) {
+                            status = 2;
+                            break;
+                        }
+                        adj_stencil_t S;
+                        adj_stencil3(after, slow, &d, i, j, k, sz, sx, sy, which, NZ, NX, NY, &S);
+                        dlds[S.cell] += l * S.wcell;
+                        for (int b = 1; b < 8; ++b)
+                            if (S.has[b]) lam[S.up[b]] += l * S.w[b];
+                    }
+        }
+        for (int c = 0; c < 8; ++c) {
+            int i = zsi + corners[c][0], j = xsi + corners[c][1], k = ysi + corners[c][2];
+            dvz += lam[((size_t)i * NX + j) * NY + k] * t_ana3(i, j, k, dz, dx, dy, zsa, xsa, ysa, 1.0);
+        }
+        dlds[scell] += dvz;
+    }
+    free(za);
+    free(xa);
+    free(ya);
+    free(lam);
+    free(hist);
+    return status;
 }

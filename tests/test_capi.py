@@ -101,3 +101,58 @@ def test_persistent_schedule_is_refused():
     st = L.fteik3d_solve(fake, 4, 4, 4, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, 0, 0, fake, None, fake,
                          fake, fake, ws, 0, None)
     assert st == _lib.ERR_UNSUPPORTED
+
+
+def test_adjoint_host_argument_checks():
+    """Adjoint and logged-solve entry points validate before touching a device."""
+    from fteikpy_amd import _lib
+
+    L = _lib.lib()
+    o = np.zeros(3)
+    op = o.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert L.fteik3d_history_size(-1, 8) == 0
+    assert L.fteik3d_history_size(100, 8) >= 100 * 12
+    assert L.fteik3d_adjoint_history_workspace_size(4, 4, 4, 2, 10) > 2 * 125 * 8
+    assert L.fteik3d_adjoint_workspace_size(4, 4, 4, 2) >= 2 * 125 * 21
+    ev = ctypes.c_longlong(0)
+    # no history buffer
+    st = L.fteik3d_solve_logged(None, 4, 4, 4, 1.0, 1.0, 1.0, op, 1, 0, 10, 0, 0, None, None,
+                                None, None, 0, None, 0, ctypes.byref(ev), 0, None)
+    assert st == _lib.ERR_ARG
+    # (source, node) indices beyond 32 bits
+    st = L.fteik3d_solve_logged(None, 1000, 1000, 1000, 1.0, 1.0, 1.0, op, 8, 0, 10, 0, 0, None,
+                                None, None, None, 0, ctypes.c_void_p(1), 1 << 20,
+                                ctypes.byref(ev), 0, None)
+    assert st == _lib.ERR_ARG
+    st = L.fteik3d_adjoint_history(None, 4, 4, 4, 1.0, 1.0, 1.0, op, op, 1, None, None, None, 0,
+                                   None, None, None, 0, None)
+    assert st == _lib.ERR_ARG
+    st = L.fteik3d_adjoint(None, 4, 4, 4, 1.0, 1.0, 1.0, op, op, 1, None, None, 0, None, None,
+                           None, None, None, 0, None)
+    assert st == _lib.ERR_ARG
+    with pytest.raises(RuntimeError, match="do not reproduce"):
+        _lib.check(_lib.ERR_UNSETTLED)
+
+
+def test_receiver_traveltimes_refuses_host_tensors():
+    import torch
+
+    import fteikpy_amd
+
+    s = torch.full((4, 4, 4), 1e-3, dtype=torch.float64)
+    with pytest.raises(ValueError, match="HIP device"):
+        fteikpy_amd.receiver_traveltimes(s, (1.0, 1.0, 1.0), [[1.0, 1.0, 1.0]], [[2.0, 2.0, 2.0]])
+    with pytest.raises(TypeError):
+        fteikpy_amd.receiver_traveltimes(s.float(), (1.0, 1.0, 1.0), [[1.0, 1.0, 1.0]],
+                                         [[2.0, 2.0, 2.0]])
+
+
+def test_history_size_round_trip():
+    """A buffer of fteik3d_history_size(n, p) bytes holds exactly n changes (the library
+    derives the capacity from the byte count)."""
+    from fteikpy_amd import _lib
+
+    L = _lib.lib()
+    for n in (1, 16, 17, 1000, 123457):
+        for p in (8, 400):
+            assert L.fteik3d_history_size(n, p) < L.fteik3d_history_size(n + 64, p)
