@@ -73,7 +73,9 @@ struct AdjArgs {
     uint8_t *code;     // [nsrc][nodes]
     double *lam;       // [nsrc][nodes]
     int *cnt;          // [nsrc][nodes] dependents not pushed yet
-    double *dlds;      // [nz][nx][ny]
+    double *dlds;      // [ngroup][nz][nx][ny] partial sums, source s adds into s % ngroup
+    long long cells;   // nz * nx * ny
+    int ngroup;
     int *stale;        // stale nodes carrying adjoint
     long long nodes;
     int nz, nx, ny, NZ, NX, NY, nsrc;
@@ -85,6 +87,11 @@ __device__ __forceinline__ long long node_at(const AdjArgs &A, int i, int j, int
 }
 __device__ __forceinline__ long long cell_at(const AdjArgs &A, int i, int j, int k) {
     return ((long long)i * A.nx + j) * A.ny + k;
+}
+// dL/ds partial sums: sources of one hyperplane bucket or frontier hit the same cells at
+// the same time, so they accumulate into separate arrays (reduced at the end).
+__device__ __forceinline__ void add_dlds(const AdjArgs &A, int s, long long cell, double v) {
+    atomicAdd(A.dlds + (long long)(s % A.ngroup) * A.cells + cell, v);
 }
 __device__ __forceinline__ void node_ijk(const AdjArgs &A, long long n, int *i, int *j, int *k) {
     *k = (int)(n % A.NY);
@@ -287,7 +294,7 @@ __global__ void adj_receiver_kernel(AdjArgs A, Axis Z, Axis X, Axis Y, const dou
     const long long scell = cell_at(A, S.p.zsi, S.p.xsi, S.p.ysi);
     const Corner cz = corner(Z, Z.cell(zq)), cx = corner(X, X.cell(xq)), cy = corner(Y, Y.cell(yq));
     if (cz.i1 == Z.cell(S.zs) && cx.i1 == X.cell(S.xs) && cy.i1 == Y.cell(S.ys)) {
-        atomicAdd(A.dlds + scell, wr * dist);
+        add_dlds(A, s, scell, wr * dist);
         return;
     }
     const double *tt = A.tt + (long long)s * A.nodes;
@@ -304,7 +311,7 @@ __global__ void adj_receiver_kernel(AdjArgs A, Axis Z, Axis X, Axis Y, const dou
         id[c] = node_at(A, cz.i1 + di, cx.i1 + dj, cy.i1 + dk);
         t[c] = tt[id[c]];
         if (t[c] == 0.0) {  // the source sits on this node (_vinterp3d.py:246-250)
-            atomicAdd(A.dlds + scell, wr * dist);
+            add_dlds(A, s, scell, wr * dist);
             return;
         }
         const double w = fabs(((di ? cz.x1 : cz.x2) - zq) * ((dj ? cx.x1 : cx.x2) - xq) *
@@ -352,7 +359,7 @@ __global__ void adj_round_kernel(AdjArgs A, const unsigned *front, const int *nf
             const SrcParam &P = A.src[s].p;
             const double a = A.dz * ((double)i - P.zsa), b = A.dx * ((double)j - P.xsa),
                          c = A.dy * ((double)k - P.ysa);
-            atomicAdd(A.dlds + cell_at(A, P.zsi, P.xsi, P.ysi), lam * sqrt(a * a + b * b + c * c));
+            add_dlds(A, s, cell_at(A, P.zsi, P.xsi, P.ysi), lam * sqrt(a * a + b * b + c * c));
         }
         return;
     }
@@ -361,7 +368,7 @@ __global__ void adj_round_kernel(AdjArgs A, const unsigned *front, const int *nf
     const double *tt = A.tt + (long long)s * A.nodes;
     Stencil S;
     stencil(A, tt, i, j, k, p, (int)which, &S);
-    if (lam != 0.0) atomicAdd(A.dlds + S.cell, lam * S.wcell);
+    if (lam != 0.0) add_dlds(A, s, S.cell, lam * S.wcell);
     const int sz = dir_sz(p), sx = dir_sx(p), sy = dir_sy(p);
     for (int b = 1; b < 8; ++b) {
         if ((b & (int)which) != b) continue;
@@ -381,20 +388,64 @@ __device__ __forceinline__ int plane_of(const AdjArgs &A, long long n, int p) {
     return a + b + c;
 }
 
-__global__ void hist_count_kernel(AdjArgs A, const unsigned *node, unsigned long long first,
-                                  unsigned n, int p, int *bins) {
-    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const unsigned g = node[first + q];
-    atomicAdd(bins + plane_of(A, (long long)(g % (unsigned long long)A.nodes), p), 1);
+// Bucketing of one pass's changes by hyperplane: few bins (nz + nx + ny + 1), many
+// entries, so each workgroup counts its kHistItems * 256 entries in LDS and touches every
+// global bin once.
+constexpr unsigned kHistUnused = 0xffffffffu;  // padding entry of a reserved chunk
+constexpr int kHistThreads = 256;
+constexpr int kHistItems = 16;
+constexpr int kMaxBins = 8192;
+
+__global__ __launch_bounds__(kHistThreads) void hist_count_kernel(
+    AdjArgs A, const unsigned *node, unsigned long long first, unsigned n, int p, int nbins,
+    int *bins) {
+    __shared__ int local[kMaxBins];
+    for (int b = threadIdx.x; b < nbins; b += kHistThreads) local[b] = 0;
+    __syncthreads();
+    const unsigned q0 = blockIdx.x * (kHistThreads * kHistItems) + threadIdx.x;
+#pragma unroll
+    for (int it = 0; it < kHistItems; ++it) {
+        const unsigned q = q0 + it * kHistThreads;
+        if (q < n) {
+            const unsigned g = node[first + q];
+            if (g != kHistUnused)
+                atomicAdd(local + plane_of(A, (long long)(g % (unsigned long long)A.nodes), p), 1);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += kHistThreads)
+        if (local[b]) atomicAdd(bins + b, local[b]);
 }
 
-__global__ void hist_scatter_kernel(AdjArgs A, const unsigned *node, unsigned long long first,
-                                    unsigned n, int p, int *cursor, unsigned *perm) {
-    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const unsigned g = node[first + q];
-    perm[atomicAdd(cursor + plane_of(A, (long long)(g % (unsigned long long)A.nodes), p), 1)] = q;
+// Same blocks as hist_count_kernel: ranks inside the block from LDS, one global range
+// reservation per (block, bin).
+__global__ __launch_bounds__(kHistThreads) void hist_scatter_kernel(
+    AdjArgs A, const unsigned *node, unsigned long long first, unsigned n, int p, int nbins,
+    int *cursor, unsigned *perm) {
+    __shared__ int local[kMaxBins];
+    for (int b = threadIdx.x; b < nbins; b += kHistThreads) local[b] = 0;
+    __syncthreads();
+    const unsigned q0 = blockIdx.x * (kHistThreads * kHistItems) + threadIdx.x;
+    int h[kHistItems], r[kHistItems];
+#pragma unroll
+    for (int it = 0; it < kHistItems; ++it) {
+        const unsigned q = q0 + it * kHistThreads;
+        h[it] = -1;
+        if (q < n) {
+            const unsigned g = node[first + q];
+            if (g != kHistUnused) {
+                h[it] = plane_of(A, (long long)(g % (unsigned long long)A.nodes), p);
+                r[it] = atomicAdd(local + h[it], 1);
+            }
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += kHistThreads)
+        if (local[b]) local[b] = atomicAdd(cursor + b, local[b]);  // block's base in bin b
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kHistItems; ++it)
+        if (h[it] >= 0) perm[local[h[it]] + r[it]] = q0 + it * kHistThreads;
 }
 
 // One hyperplane of pass p, reversed: push the adjoint of every node it changed, restore
@@ -418,7 +469,7 @@ __global__ void hist_reverse_kernel(AdjArgs A, const unsigned *node, const doubl
         if (m != tt[n]) atomicAdd(mismatch, 1);
         Stencil S;
         stencil(A, tt, i, j, k, p, which, &S);
-        atomicAdd(A.dlds + S.cell, lam * S.wcell);
+        add_dlds(A, s, S.cell, lam * S.wcell);
         const int sz = dir_sz(p), sx = dir_sx(p), sy = dir_sy(p);
         for (int b = 1; b < 8; ++b)
             if ((b & which) == b && S.w[b] != 0.0)
@@ -440,7 +491,16 @@ __global__ void hist_seed_kernel(AdjArgs A) {
     if (lam == 0.0) return;
     const double a = A.dz * ((double)i - P.zsa), b = A.dx * ((double)j - P.xsa),
                  cc = A.dy * ((double)k - P.ysa);
-    atomicAdd(A.dlds + cell_at(A, P.zsi, P.xsi, P.ysi), lam * sqrt(a * a + b * b + cc * cc));
+    add_dlds(A, s, cell_at(A, P.zsi, P.xsi, P.ysi), lam * sqrt(a * a + b * b + cc * cc));
+}
+
+// dlds = sum of the partial arrays, in group order.
+__global__ void dlds_reduce_kernel(const double *part, int ngroup, long long cells, double *dlds) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    double v = 0.0;
+    for (int g = 0; g < ngroup; ++g) v += part[(long long)g * cells + c];
+    dlds[c] = v;
 }
 
 // Clears a frontier counter on the device.
@@ -473,9 +533,13 @@ using namespace fteik;
 
 namespace {
 
+int dlds_groups(int nsrc) { return std::min(nsrc, 32); }
+
 void fill_args(AdjArgs &A, const double *slow, int nz, int nx, int ny, double dz, double dx,
                double dy, const double *tt, int nsrc) {
     A.slow = slow; A.tt = tt; A.nodes = (long long)(nz + 1) * (nx + 1) * (ny + 1);
+    A.cells = (long long)nz * nx * ny;
+    A.ngroup = dlds_groups(nsrc);
     A.nz = nz; A.nx = nx; A.ny = ny; A.NZ = nz + 1; A.NX = nx + 1; A.NY = ny + 1; A.nsrc = nsrc;
     A.dz = dz; A.dx = dx; A.dy = dy;
     A.dz2i = 1.0 / dz / dz; A.dx2i = 1.0 / dx / dx; A.dy2i = 1.0 / dy / dy;
@@ -507,8 +571,10 @@ extern "C" size_t fteik3d_adjoint_history_workspace_size(int nz, int nx, int ny,
     if (nz < 1 || nx < 1 || ny < 1 || nsrc < 1 || events < 0) return 0;
     const size_t total = (size_t)(nz + 1) * (nx + 1) * (ny + 1) * nsrc;
     const size_t nbins = (size_t)(nz + nx + ny + 1);
+    const size_t part = (size_t)dlds_groups(nsrc) * nz * nx * ny * sizeof(double);
     return align256(sizeof(AdjSrc) * nsrc) + align256(total * sizeof(double)) +
-           2 * align256(nbins * sizeof(int)) + align256((size_t)events * sizeof(unsigned)) + 256;
+           2 * align256(nbins * sizeof(int)) + align256((size_t)events * sizeof(unsigned)) +
+           align256(part) + 256;
 }
 
 extern "C" int fteik3d_adjoint_history(const double *slow, int nz, int nx, int ny, double dz,
@@ -541,6 +607,7 @@ extern "C" int fteik3d_adjoint_history(const double *slow, int nz, int nx, int n
 
     const long long total = nodes * nsrc;
     const int nbins = nz + nx + ny + 1;
+    if (nbins > kMaxBins) return FTEIK_ERR_ARG;  // hyperplanes per pass beyond the LDS bins
     char *p = (char *)workspace;
     AdjSrc *dsrc = (AdjSrc *)p;
     p += align256(sizeof(AdjSrc) * nsrc);
@@ -552,19 +619,21 @@ extern "C" int fteik3d_adjoint_history(const double *slow, int nz, int nx, int n
     p += align256(nbins * sizeof(int));
     unsigned *perm = (unsigned *)p;
     p += align256((size_t)events * sizeof(unsigned));
+    double *part = (double *)p;
+    p += align256((size_t)dlds_groups(nsrc) * nz * nx * ny * sizeof(double));
     int *mismatch = (int *)p;
 
     AdjArgs A{};
     fill_args(A, slow, nz, nx, ny, dz, dx, dy, tt, nsrc);
     A.src = dsrc;
     A.lam = lam;
-    A.dlds = dlds;
+    A.dlds = part;
     const Axis Z{origin[0], dz, nz + 1}, X{origin[1], dx, nx + 1}, Y{origin[2], dy, ny + 1};
 
     FTEIK_CHECK(hipMemcpyAsync(dsrc, hs.data(), sizeof(AdjSrc) * nsrc, hipMemcpyHostToDevice, stream));
     FTEIK_CHECK(hipMemsetAsync(lam, 0, total * sizeof(double), stream));
     FTEIK_CHECK(hipMemsetAsync(mismatch, 0, sizeof(int), stream));
-    FTEIK_CHECK(hipMemsetAsync(dlds, 0, (size_t)nz * nx * ny * sizeof(double), stream));
+    FTEIK_CHECK(hipMemsetAsync(part, 0, (size_t)A.ngroup * A.cells * sizeof(double), stream));
     if (nrec > 0) {
         const long long nr = (long long)nrec * nsrc;
         hipLaunchKernelGGL(adj_receiver_kernel, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0,
@@ -579,10 +648,10 @@ extern "C" int fteik3d_adjoint_history(const double *slow, int nz, int nx, int n
         if (last <= first) continue;
         const unsigned n = (unsigned)(last - first);
         const int pdir = P % 8;
-        const unsigned nb = (n + 255) / 256;
+        const unsigned nb = (n + kHistThreads * kHistItems - 1) / (kHistThreads * kHistItems);
         FTEIK_CHECK(hipMemsetAsync(bins, 0, nbins * sizeof(int), stream));
-        hipLaunchKernelGGL(hist_count_kernel, dim3(nb), dim3(256), 0, stream, A, hv.node, first, n,
-                           pdir, bins);
+        hipLaunchKernelGGL(hist_count_kernel, dim3(nb), dim3(kHistThreads), 0, stream, A, hv.node,
+                           first, n, pdir, nbins, bins);
         FTEIK_CHECK(hipGetLastError());
         FTEIK_CHECK(hipMemcpyAsync(counts.data(), bins, nbins * sizeof(int), hipMemcpyDeviceToHost,
                                    stream));
@@ -595,8 +664,8 @@ extern "C" int fteik3d_adjoint_history(const double *slow, int nz, int nx, int n
         }
         FTEIK_CHECK(hipMemcpyAsync(cursor, start.data(), nbins * sizeof(int), hipMemcpyHostToDevice,
                                    stream));
-        hipLaunchKernelGGL(hist_scatter_kernel, dim3(nb), dim3(256), 0, stream, A, hv.node, first,
-                           n, pdir, cursor, perm);
+        hipLaunchKernelGGL(hist_scatter_kernel, dim3(nb), dim3(kHistThreads), 0, stream, A,
+                           hv.node, first, n, pdir, nbins, cursor, perm);
         FTEIK_CHECK(hipGetLastError());
         for (int h = nbins - 1; h >= 0; --h) {
             const int c = counts[h];
@@ -609,6 +678,9 @@ extern "C" int fteik3d_adjoint_history(const double *slow, int nz, int nx, int n
     }
     hipLaunchKernelGGL(hist_seed_kernel, dim3((nsrc * 8 + 255) / 256), dim3(256), 0, stream, A);
     FTEIK_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(dlds_reduce_kernel, dim3((unsigned)((A.cells + 255) / 256)), dim3(256), 0,
+                       stream, part, A.ngroup, A.cells, dlds);
+    FTEIK_CHECK(hipGetLastError());
     int bad = 0;
     FTEIK_CHECK(hipMemcpyAsync(&bad, mismatch, sizeof(int), hipMemcpyDeviceToHost, stream));
     FTEIK_CHECK(hipStreamSynchronize(stream));
@@ -620,8 +692,10 @@ extern "C" int fteik3d_adjoint_history(const double *slow, int nz, int nx, int n
 extern "C" size_t fteik3d_adjoint_workspace_size(int nz, int nx, int ny, int nsrc) {
     if (nz < 1 || nx < 1 || ny < 1 || nsrc < 1) return 0;
     const size_t total = (size_t)(nz + 1) * (nx + 1) * (ny + 1) * nsrc;
+    const size_t part = (size_t)dlds_groups(nsrc) * nz * nx * ny * sizeof(double);
     return align256(sizeof(AdjSrc) * nsrc) + align256(total) + align256(total * sizeof(double)) +
-           align256(total * sizeof(int)) + 2 * align256(total * sizeof(unsigned)) + 256;
+           align256(total * sizeof(int)) + 2 * align256(total * sizeof(unsigned)) +
+           align256(part) + 256;
 }
 
 extern "C" int fteik3d_adjoint(const double *slow, int nz, int nx, int ny, double dz, double dx,
@@ -637,17 +711,8 @@ extern "C" int fteik3d_adjoint(const double *slow, int nz, int nx, int ny, doubl
     const long long total = nodes * nsrc;
     if (total > 0xffffffffLL) return FTEIK_ERR_ARG;  // frontier entries are 32-bit
     if (workspace_bytes < fteik3d_adjoint_workspace_size(nz, nx, ny, nsrc)) return FTEIK_ERR_WORKSPACE;
-    std::vector<AdjSrc> hs(nsrc);
-    for (int s = 0; s < nsrc; ++s) {
-        const double *q = src + 3 * s;
-        // relative position exactly as the solve receives it (sources - origin)
-        if (!place_source3(q[0] - origin[0], q[1] - origin[1], q[2] - origin[2], nz, nx, ny, dz,
-                           dx, dy, &hs[s].p))
-            return FTEIK_ERR_SOURCE_OOB;
-        hs[s].zs = q[0];
-        hs[s].xs = q[1];
-        hs[s].ys = q[2];
-    }
+    std::vector<AdjSrc> hs;
+    if (int st = place_sources(origin, src, nsrc, nz, nx, ny, dz, dx, dy, hs)) return st;
     hipStream_t stream = (hipStream_t)stream_;
     char *p = (char *)workspace;
     AdjSrc *dsrc = (AdjSrc *)p;
@@ -662,23 +727,20 @@ extern "C" int fteik3d_adjoint(const double *slow, int nz, int nx, int ny, doubl
     p += align256(total * sizeof(unsigned));
     unsigned *fb = (unsigned *)p;
     p += align256(total * sizeof(unsigned));
+    double *part = (double *)p;
+    p += align256((size_t)dlds_groups(nsrc) * nz * nx * ny * sizeof(double));
     int *ctr = (int *)p;  // [0], [1]: frontier sizes; [2]: stale count
 
-    AdjArgs A;
-    A.slow = slow; A.tt = tt; A.src = dsrc; A.code = code; A.lam = lam; A.cnt = cnt;
-    A.dlds = dlds; A.stale = ctr + 2; A.nodes = nodes;
-    A.nz = nz; A.nx = nx; A.ny = ny; A.NZ = nz + 1; A.NX = nx + 1; A.NY = ny + 1; A.nsrc = nsrc;
-    A.dz = dz; A.dx = dx; A.dy = dy;
-    A.dz2i = 1.0 / dz / dz; A.dx2i = 1.0 / dx / dx; A.dy2i = 1.0 / dy / dy;
-    A.dz2dx2 = A.dz2i * A.dx2i; A.dz2dy2 = A.dz2i * A.dy2i; A.dx2dy2 = A.dx2i * A.dy2i;
-    A.dsum = A.dz2i + A.dx2i + A.dy2i;
+    AdjArgs A{};
+    fill_args(A, slow, nz, nx, ny, dz, dx, dy, tt, nsrc);
+    A.src = dsrc; A.code = code; A.lam = lam; A.cnt = cnt; A.dlds = part; A.stale = ctr + 2;
     const Axis Z{origin[0], dz, nz + 1}, X{origin[1], dx, nx + 1}, Y{origin[2], dy, ny + 1};
 
     FTEIK_CHECK(hipMemcpyAsync(dsrc, hs.data(), sizeof(AdjSrc) * nsrc, hipMemcpyHostToDevice, stream));
     FTEIK_CHECK(hipMemsetAsync(lam, 0, total * sizeof(double), stream));
     FTEIK_CHECK(hipMemsetAsync(cnt, 0, total * sizeof(int), stream));
     FTEIK_CHECK(hipMemsetAsync(ctr, 0, 4 * sizeof(int), stream));
-    FTEIK_CHECK(hipMemsetAsync(dlds, 0, (size_t)nz * nx * ny * sizeof(double), stream));
+    FTEIK_CHECK(hipMemsetAsync(part, 0, (size_t)A.ngroup * A.cells * sizeof(double), stream));
     const unsigned nb = (unsigned)((total + 255) / 256);
     hipLaunchKernelGGL(adj_select_kernel, dim3(nb), dim3(256), 0, stream, A);
     FTEIK_CHECK(hipGetLastError());
@@ -707,6 +769,9 @@ extern "C" int fteik3d_adjoint(const double *slow, int nz, int nx, int ny, doubl
         cur = 1 - cur;
         ++rounds;
     }
+    hipLaunchKernelGGL(dlds_reduce_kernel, dim3((unsigned)((A.cells + 255) / 256)), dim3(256), 0,
+                       stream, part, A.ngroup, A.cells, dlds);
+    FTEIK_CHECK(hipGetLastError());
     int stale = 0;
     FTEIK_CHECK(hipMemcpyAsync(&stale, ctr + 2, sizeof(int), hipMemcpyDeviceToHost, stream));
     FTEIK_CHECK(hipStreamSynchronize(stream));

@@ -4,7 +4,8 @@
 //
 // Layout of the caller's buffer (all offsets 256-byte aligned):
 //   HistHeader | seg[max_pass + 1] (u64: entries before pass P) | node[cap] (u32: source *
-//   nodes + node) | old[cap] (f64)
+//   nodes + node, or 0xffffffff for an unused entry) | old[cap] (f64)
+// Writers reserve entries in chunks, so a pass's segment holds unused entries too.
 #pragma once
 
 #include <stddef.h>

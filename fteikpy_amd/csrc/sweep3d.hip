@@ -120,6 +120,16 @@ __device__ __forceinline__ void decode_brick(int d, int t, int nba, int nbb, int
 
 // One brick for W sources (wave w handles list[g0 + w]).  Every early return is uniform
 // across the workgroup.
+// Logged solve: a wave reserves history entries kLogChunk at a time (one counter update
+// per chunk); the unused tail of a chunk is filled with kLogUnused, which the reverse skips.
+constexpr unsigned kLogChunk = 256;
+constexpr unsigned kLogUnused = 0xffffffffu;
+__device__ __forceinline__ void log_pad(const SweepArgs &A, unsigned long long pos,
+                                        unsigned long long end, int lane) {
+    for (unsigned long long e = pos + lane; e < end; e += 64)
+        if (e < (unsigned long long)A.log_cap) A.log_node[e] = kLogUnused;
+}
+
 template <bool GRAD, bool LOG, int W>
 __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int g0, int nact) {
     constexpr int kThreads = 64 * W;
@@ -288,6 +298,8 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
     const uint8_t code_z = sgn_code(sz), code_x = (uint8_t)(sgn_code(sx) << 2),
                   code_y = (uint8_t)(sgn_code(sy) << 4);
 
+    // LOG: this wave's reserved history chunk [log_pos, log_end)
+    unsigned long long log_pos = 0, log_end = 0;
     unsigned hist = 0;   // bit k: this lane changed its node k steps ago
     unsigned chist = 0;  // bit k: this lane changed its node's change bits k steps ago
     bool any_change = false;
@@ -343,17 +355,20 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
             chist = (chist << 1) | (cchanged ? 1u : 0u);
             any_change |= changed_now;
             if (LOG) {
-                // append this wave's changes: one counter update per wave, entries in lane
-                // order
+                // append this wave's changes in lane order into its reserved chunk
                 const unsigned long long m = __ballot(changed_now);
                 if (m) {
-                    const int leader = __ffsll((long long)m) - 1;
-                    unsigned long long base = 0;
-                    if (lane == leader) base = atomicAdd(A.log_count, (unsigned long long)__popcll(m));
-                    base = __shfl(base, leader);
+                    const unsigned cnt = (unsigned)__popcll(m);
+                    if (log_end - log_pos < cnt) {
+                        log_pad(A, log_pos, log_end, lane);
+                        unsigned long long base = 0;
+                        if (lane == 0) base = atomicAdd(A.log_count, (unsigned long long)kLogChunk);
+                        log_pos = __shfl(base, 0);
+                        log_end = log_pos + kLogChunk;
+                    }
                     if (changed_now) {
                         const unsigned long long e =
-                            base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+                            log_pos + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
                         if (e < (unsigned long long)A.log_cap) {
                             A.log_node[e] = (unsigned)((long long)src * A.node_stride +
                                                        (long long)node_i(a_first + p) * plane_nodes +
@@ -361,6 +376,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
                             A.log_old[e] = t_prev;
                         }
                     }
+                    log_pos += cnt;
                 }
             }
 
@@ -383,6 +399,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
         }
     }
 
+    if (LOG) log_pad(A, log_pos, log_end, lane);
     if (has_src) {
         const bool wave_changed = __any(any_change);
         if (lane == 0) {
@@ -1003,8 +1020,8 @@ extern "C" int fteik3d_solve_logged(const double *slow, int nz, int nx, int ny, 
                                     int use_graph, void *stream) {
     if (!history || nz < 1 || nx < 1 || ny < 1 || nsrc < 1) return FTEIK_ERR_ARG;
     if (nsweep <= 0 && max_sets < 1) return FTEIK_ERR_ARG;
-    // logged node indices are 32-bit
-    if ((long long)(nz + 1) * (nx + 1) * (ny + 1) * nsrc > 0xffffffffLL) return FTEIK_ERR_ARG;
+    // logged node indices are 32-bit (0xffffffff marks an unused entry)
+    if ((long long)(nz + 1) * (nx + 1) * (ny + 1) * nsrc >= 0xffffffffLL) return FTEIK_ERR_ARG;
     const int max_pass = 8 * (nsweep > 0 ? nsweep : max_sets);
     const long long cap = hist_capacity(history_bytes, max_pass);
     if (cap < 1) return FTEIK_ERR_WORKSPACE;

@@ -80,3 +80,26 @@ def model_3d_inclusions(seed=1003, shape=(160, 320, 320), d=12.5, nsrc_side=16, 
     ry = (np.arange(nrec_side) + 0.5) * ey / nrec_side
     rec = np.stack(np.meshgrid(np.zeros(1), rx, ry, indexing="ij"), -1).reshape(-1, 3)
     return v, (d, d, d), src, rec
+
+
+def model_3d_smooth_slowness(seed=1005, shape=(128, 256, 256), d=10.0, sigma=4.0, amp=0.08,
+                             src_grid=(8, 16), rec_grid=(32, 32)):
+    """Gradient workload: true slowness smooth random (+-amp around 1/3000 s/m; N(0,1)
+    filtered with `sigma` cells, scaled to max |p| = 1) and the starting model, its 1D
+    (depth) average.  Sources on a src_grid surface grid at z = d/2, receivers on a
+    rec_grid grid at z = 0.  Returns (true, start, (d, d, d), src, rec)."""
+    rng = np.random.default_rng(seed)
+    p = gaussian_filter_reflect(rng.standard_normal(shape), sigma)
+    p /= np.abs(p).max()
+    true = (1.0 / 3000.0) * (1.0 + amp * p)
+    start = np.ascontiguousarray(
+        np.broadcast_to(true.mean(axis=(1, 2))[:, None, None], shape))
+    nz, nx, ny = shape
+    ex, ey = nx * d, ny * d
+    sx = (np.arange(src_grid[0]) + 0.5) * ex / src_grid[0]
+    sy = (np.arange(src_grid[1]) + 0.5) * ey / src_grid[1]
+    src = np.stack(np.meshgrid(np.full(1, d / 2), sx, sy, indexing="ij"), -1).reshape(-1, 3)
+    rx = (np.arange(rec_grid[0]) + 0.5) * ex / rec_grid[0]
+    ry = (np.arange(rec_grid[1]) + 0.5) * ey / rec_grid[1]
+    rec = np.stack(np.meshgrid(np.zeros(1), rx, ry, indexing="ij"), -1).reshape(-1, 3)
+    return true, start, (d, d, d), src, rec
