@@ -1,11 +1,13 @@
-"""Benchmark: converged FP64 3D source-grids per second, batched vs plain, on one GPU.
+"""Benchmark: converged FP64 3D source-grids per second, batched vs plain, on one GPU; and
+seconds per forward+adjoint iteration of the gradient workload.
 
 Workload (default): 256 sources on a 160 x 320 x 320 FP64 model, 12.5 m spacing, linear
 gradient 2000 + 0.8 z m/s with 20 spherical inclusions (+-15%); sources on a 16 x 16
 surface grid at 6.25 m depth.  A step is one batched solve of all sources swept to
 convergence (sets repeat until one changes no node), sweep sets replayed from a hipGraph.
 
-"plain" is the per-source solve looped over a subset of the sources.  The CPU baseline is
+"plain" is the per-source solve looped over a subset of the sources.  The gradient
+workload (fwd_adjoint in the output) is described in fwd_adjoint().  The CPU baseline is
 the oracle (C restatement of the reference's numba kernels, OpenMP over sources like the
 reference's prange) timed on this host on a bounded subset of the same sources.
 Prints one JSON line.
@@ -39,6 +41,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sources", type=int, default=0, help="0 = one per host core (max 16)")
     ap.add_argument("--cpu-sets", type=int, default=4, help="sweep sets timed on the CPU")
+    ap.add_argument("--adj-steps", type=int, default=3,
+                    help="timed forward+adjoint iterations of the gradient workload (0 = skip)")
+    ap.add_argument("--adj-shape", type=str, default="128,256,256")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -97,6 +102,10 @@ def main():
         plain_s = time.perf_counter() - tp
         plain_equal = bool(np.array_equal(plain_tt0.cpu().numpy(), batched_tt0))
 
+    del out, tt
+    torch.cuda.empty_cache()
+    adj = fwd_adjoint(args, torch, fteikpy_amd, synthetic, dev) if args.adj_steps > 0 else None
+
     nodes = nz * nx * ny
     alg_bytes = sets_total * 8 * nodes * BYTES_PER_NSU
     achieved = alg_bytes / gpu_s / 1e9
@@ -123,7 +132,7 @@ def main():
                          "path, OpenMP over sources"}
 
     res = {
-        "metric": "converged FP64 source-grids/sec 3D, batched vs plain GPU",
+        "metric": "converged FP64 source-grids/sec 3D, batched vs plain GPU; fwd+adjoint s/iter",
         "value": value,
         "unit": "source-grids/s",
         "n_gpus": 1,
@@ -149,8 +158,63 @@ def main():
                      "note": "24 B per node update x 8 passes x nodes x sets, over HIP-event "
                              "time of the timed solves"},
         "cpu_baseline": cpu,
+        "fwd_adjoint": adj,
     }
     print(json.dumps(res), flush=True)
+
+
+def fwd_adjoint(args, torch, fa, synthetic, dev):
+    """Gradient workload: FP64 slowness 128 x 256 x 256, d = 10 m, true model smooth random
+    (+-8% around 1/3000 s/m), gradient at its 1D average; 128 surface sources x 1024
+    receivers; loss 0.5 sum (T - T_obs)^2.  One iteration = converged forward of all
+    sources (with its sweep history), receiver interpolation, loss, exact adjoint, dL/ds."""
+    from fteikpy_amd import _adjoint
+
+    shape = tuple(int(s) for s in args.adj_shape.split(","))
+    true, start, d, src, rec = synthetic.model_3d_smooth_slowness(shape=shape)
+    st = torch.tensor(true, device=dev)
+    s0 = torch.tensor(start, device=dev)
+    with torch.no_grad():
+        tobs = fa.receiver_traveltimes(st, d, src, rec)
+    del st
+
+    def iteration(stats=None):
+        s = s0.clone().requires_grad_(True)
+        T = fa.receiver_traveltimes(s, d, src, rec, stats=stats)
+        loss = 0.5 * ((T - tobs) ** 2).sum()
+        loss.backward()
+        return s.grad
+
+    iteration()
+    torch.cuda.synchronize()
+    times, stats = [], {}
+    for _ in range(args.adj_steps):
+        t0 = time.perf_counter()
+        g = iteration(stats)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    # the plain forward (no history) of the same sources, for the ratio
+    rec_dev = torch.as_tensor(rec, device=dev)
+    fwd = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        tt, vz, _, _, _ = _adjoint._solve(s0, d, src, 200, False)
+        _adjoint._interp(tt, d, np.zeros(3), src, vz, rec_dev)
+        torch.cuda.synchronize()
+        fwd.append(time.perf_counter() - t0)
+        del tt
+    s_iter = float(np.median(times))
+    f = float(min(fwd))
+    nodes = np.prod(np.array(shape) + 1) * len(src)
+    return {"s_per_iter": s_iter, "iters": args.adj_steps, "forward_s": f,
+            "ratio_to_forward": s_iter / f, "mode": "exact",
+            "sets_per_source": [int(stats["sets"].min()), int(stats["sets"].max())],
+            "history_entries_per_node": stats["events"] / nodes,
+            "grad_finite": bool(torch.isfinite(g).all()),
+            "config": {"workload": "adjoint-state gradient, smooth random slowness (+-8%), "
+                                   "gradient at its 1D average",
+                       "grid": list(shape), "spacing_m": d[0], "sources": len(src),
+                       "receivers": len(rec), "loss": "0.5 sum (T - T_obs)^2"}}
 
 
 if __name__ == "__main__":
