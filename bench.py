@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="batched brick height (0 = default)")
     ap.add_argument("--shape", type=str, default="160,320,320")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (brick height and group from the library defaults)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sources", type=int, default=0, help="0 = one per host core (max 16)")
     ap.add_argument("--cpu-sets", type=int, default=4, help="sweep sets timed on the CPU")
@@ -67,7 +69,12 @@ def main():
     stream = torch.cuda.Stream(device=dev)
     use_graph = not args.no_graph
 
+    tune = None
     with torch.cuda.stream(stream):
+        if not args.no_tune and not args.tile:
+            # brick height and group for this workload, timed on it before the warm-up
+            tune = fteikpy_amd.autotune.tune3d(eik, src, tiles=(16, 32, 48, 0),
+                                               use_graph=use_graph)
         out = torch.empty((nsrc, nz + 1, nx + 1, ny + 1), dtype=torch.float64, device=dev)
         for _ in range(args.warmup):
             eik.solve_tensors(src, nsweep=None, tile_a=args.tile, use_graph=use_graph, out=out)
@@ -148,7 +155,10 @@ def main():
                                "inclusions, converged",
                    "grid": list(shape), "spacing_m": d[0], "sources": nsrc,
                    "sets_per_source": [int(k_sets.min()), int(k_sets.max())],
-                   "hipgraph": use_graph, "tile_a": args.tile},
+                   "hipgraph": use_graph,
+                   "tile_a": args.tile or (tune["tile_a"] if tune else 0),
+                   "group": tune["group"] if tune else 0},
+        "autotune": tune,
         "plain_value": nplain / plain_s,
         "plain_sources_timed": nplain,
         "batched_over_plain": value / (nplain / plain_s),
@@ -174,6 +184,10 @@ def fwd_adjoint(args, torch, fa, synthetic, dev):
     true, start, d, src, rec = synthetic.model_3d_smooth_slowness(shape=shape)
     st = torch.tensor(true, device=dev)
     s0 = torch.tensor(start, device=dev)
+    tune = None
+    if not args.no_tune:
+        # the logged forward has the sweep structure of the plain one: tune on the plain
+        tune = fa.autotune.tune3d(fa.Eikonal3D(1.0 / start, d), src)
     with torch.no_grad():
         tobs = fa.receiver_traveltimes(st, d, src, rec)
     del st
@@ -211,6 +225,7 @@ def fwd_adjoint(args, torch, fa, synthetic, dev):
             "sets_per_source": [int(stats["sets"].min()), int(stats["sets"].max())],
             "history_entries_per_node": stats["events"] / nodes,
             "grad_finite": bool(torch.isfinite(g).all()),
+            "autotune": tune,
             "config": {"workload": "adjoint-state gradient, smooth random slowness (+-8%), "
                                    "gradient at its 1D average",
                        "grid": list(shape), "spacing_m": d[0], "sources": len(src),

@@ -4,7 +4,9 @@ Drop-in names of fteikpy/__init__.py:1-19.  Solves, interpolation, gradients and
 tracing run as hand-written HIP kernels (gfx950) through the C ABI of
 include/fteik_amd.h; arrays live in device memory as torch tensors.
 """
+from . import autotune
 from ._adjoint import receiver_traveltimes
+from ._batch import solve_batch, solve_batch_tensors
 from ._grid import Grid2D, Grid3D, TraveltimeGrid2D, TraveltimeGrid3D
 from ._helpers import get_num_threads, set_num_threads
 from ._lib import build as _build_lib
@@ -40,6 +42,9 @@ __all__ = [
     "grid_to_meshio",
     "ray_to_meshio",
     "receiver_traveltimes",
+    "solve_batch",
+    "solve_batch_tensors",
+    "autotune",
     "build",
     "__version__",
 ]

@@ -20,7 +20,7 @@ import ctypes
 
 import numpy as np
 
-from . import _lib
+from . import _lib, autotune
 from ._grid import _ptr, _torch, stream_ptr
 
 # changes per node and source a converged solve makes (measured: 2.1 - 2.8); the history
@@ -52,10 +52,11 @@ def _solve(slow, gridsize, rel_src, max_sets, logged):
     L = _lib.lib()
     wsb = L.fteik3d_workspace_size(nz, nx, ny, n)
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    tile_a, group = autotune.lookup3d((nz, nx, ny), n)
     if not logged:
-        st = L.fteik3d_solve(_ptr(slow), nz, nx, ny, *gridsize, rp, n, 0, int(max_sets), 0, 0, 0,
-                             1, _ptr(tt), None, _ptr(vzero), _ptr(sets), _ptr(ws), wsb, 0,
-                             stream_ptr())
+        st = L.fteik3d_solve(_ptr(slow), nz, nx, ny, *gridsize, rp, n, 0, int(max_sets), 0,
+                             tile_a, group, 1, _ptr(tt), None, _ptr(vzero), _ptr(sets), _ptr(ws),
+                             wsb, 0, stream_ptr())
         _lib.check(st)
         return tt, vzero.cpu().numpy(), sets.cpu().numpy(), None, 0
     max_pass = 8 * int(max_sets)
@@ -64,8 +65,8 @@ def _solve(slow, gridsize, rel_src, max_sets, logged):
         hb = L.fteik3d_history_size(cap, max_pass)
         hist = torch.empty(hb, dtype=torch.uint8, device=dev)
         ev = ctypes.c_longlong(0)
-        st = L.fteik3d_solve_logged(_ptr(slow), nz, nx, ny, *gridsize, rp, n, 0, int(max_sets), 0,
-                                    0, _ptr(tt), _ptr(vzero), _ptr(sets), _ptr(ws), wsb,
+        st = L.fteik3d_solve_logged(_ptr(slow), nz, nx, ny, *gridsize, rp, n, 0, int(max_sets),
+                                    tile_a, group, _ptr(tt), _ptr(vzero), _ptr(sets), _ptr(ws), wsb,
                                     _ptr(hist), hb, ctypes.byref(ev), 0, stream_ptr())
         if st == _lib.ERR_WORKSPACE and ev.value > cap:
             cap = ev.value + 1024

@@ -31,6 +31,20 @@ ERR_TIMEOUT = 6
 ERR_UNSUPPORTED = 7
 ERR_UNSETTLED = 8
 
+
+class Problem(ctypes.Structure):
+    """ctypes mirror of ``fteik_problem`` (include/fteik_amd.h): one model of a batch."""
+
+    _fields_ = [("ndim", ctypes.c_int), ("nz", ctypes.c_int), ("nx", ctypes.c_int),
+                ("ny", ctypes.c_int), ("dz", ctypes.c_double), ("dx", ctypes.c_double),
+                ("dy", ctypes.c_double), ("slow", ctypes.c_void_p),
+                ("src", ctypes.POINTER(ctypes.c_double)), ("nsrc", ctypes.c_int),
+                ("tile_a", ctypes.c_int), ("tt", ctypes.c_void_p), ("sgn", ctypes.c_void_p),
+                ("vzero", ctypes.c_void_p), ("sets_done", ctypes.c_void_p)]
+
+
+_pp = ctypes.POINTER(Problem)
+
 # name -> (restype, argtypes); every entry point of include/fteik_amd.h
 SIGNATURES = {
     "fteik_version": (ctypes.c_char_p, []),
@@ -54,6 +68,9 @@ SIGNATURES = {
     "fteik3d_adjoint": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _dp, _dp,
                                  _c_int, _vp, _vp, _c_int, _vp, _vp, ctypes.POINTER(_c_int),
                                  ctypes.POINTER(_c_int), _vp, ctypes.c_size_t, _vp]),
+    "fteik_batch_workspace_size": (ctypes.c_size_t, [_pp, _c_int]),
+    "fteik_solve_batch": (_c_int, [_pp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp,
+                                   ctypes.c_size_t, _c_int, ctypes.POINTER(_c_int), _vp]),
     "fteik2d_workspace_size": (ctypes.c_size_t, [_c_int]),
     "fteik2d_solve": (_c_int, [_vp, _c_int, _c_int, _c_dbl, _c_dbl, _dp, _c_int, _c_int, _c_int,
                                _c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),

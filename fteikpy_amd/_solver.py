@@ -6,6 +6,8 @@ for a batch of sources).  Keyword-only extensions:
 - ``nsweep=None`` (or <= 0): sweep until a set changes no node, at most ``max_sets`` sets;
 - ``tile_a``: z-extent of the 3D sweep bricks (0 = library default);
 - ``group``: sources per workgroup sharing the staged slowness (0 = default; 1, 2, 4, 8);
+  with both left at 0, a choice recorded by ``fteikpy_amd.autotune.tune3d`` for this grid
+  shape and number of sources is used;
 - ``schedule``: "diagonal" (one launch per brick anti-diagonal and pass; the default) or
   "persistent" (one launch per sweep set; currently refused);
 - ``use_graph``: capture one 3D sweep set as a hipGraph and replay it.
@@ -14,7 +16,7 @@ import ctypes
 
 import numpy as np
 
-from . import _lib
+from . import _lib, autotune
 from ._grid import (BaseGrid2D, BaseGrid3D, TraveltimeGrid2D, TraveltimeGrid3D, _ptr, _torch,
                     device, stream_ptr)
 
@@ -115,6 +117,8 @@ class Eikonal3D(BaseGrid3D, _SlownessCache):
         wsb = L.fteik3d_workspace_size(nz, nx, ny, n)
         ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
         slow = self._slowness()
+        if not tile_a and not group:
+            tile_a, group = autotune.lookup3d(self.shape, n)
         args = (_ptr(slow), nz, nx, ny, *self._gridsize, rp, n, _nsweep(nsweep), int(max_sets),
                 int(bool(return_gradient)), int(tile_a), int(group),
                 {"persistent": 0, "diagonal": 1}[schedule], _ptr(tt),

@@ -133,14 +133,11 @@ __global__ void fill2d_kernel(double *tt, long long n, uint8_t *sgn) {
     }
 }
 
-// Source initialisation (_fteik2d.py:355-617) and the sweep sets, one lane per source.
-__global__ void fteik2d_kernel(const double *slow, int nz, int nx, double dz, double dx,
-                               const SrcParam *prm, int nsrc, int nsweep, int max_sets,
-                               double *tt_all, uint8_t *sgn_all, double *vzero_out,
-                               int *sets_done) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nsrc) return;
-    const SrcParam P = prm[s];
+// Source initialisation (_fteik2d.py:355-617) and the sweep sets of source s, by one lane.
+__device__ void solve_source2(const double *slow, int nz, int nx, double dz, double dx,
+                              const SrcParam &P, int s, int nsweep, int max_sets,
+                              double *tt_all, uint8_t *sgn_all, double *vzero_out,
+                              int *sets_done) {
     Grid2 G;
     G.slow = slow;
     G.nz = nz;
@@ -297,6 +294,44 @@ __global__ void fteik2d_kernel(const double *slow, int nz, int nx, double dz, do
     sets_done[s] = done;
 }
 
+__global__ void fteik2d_kernel(const double *slow, int nz, int nx, double dz, double dx,
+                               const SrcParam *prm, int nsrc, int nsweep, int max_sets,
+                               double *tt_all, uint8_t *sgn_all, double *vzero_out,
+                               int *sets_done) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsrc) return;
+    solve_source2(slow, nz, nx, dz, dx, prm[s], s, nsweep, max_sets, tt_all, sgn_all, vzero_out,
+                  sets_done);
+}
+
+// Batch of 2D models: lane t takes source t - cum[m] of the model m with
+// cum[m] <= t < cum[m+1].
+struct Prob2 {
+    const double *slow;
+    int nz, nx;
+    double dz, dx;
+    const SrcParam *prm;
+    double *tt;
+    uint8_t *sgn;
+    double *vzero;
+    int *sets_done;
+};
+__global__ void fteik2d_batch_kernel(const Prob2 *probs, const int *cum, int nmodel, int nsweep,
+                                     int max_sets) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cum[nmodel]) return;
+    int lo = 0, hi = nmodel - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cum[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const Prob2 Q = probs[lo];
+    const int s = t - cum[lo];
+    solve_source2(Q.slow, Q.nz, Q.nx, Q.dz, Q.dx, Q.prm[s], s, nsweep, max_sets, Q.tt, Q.sgn,
+                  Q.vzero, Q.sets_done);
+}
+
 // 2D gradient post-pass, _fteik2d.py:623-639 (seeded analytic components survive where
 // no upwind sign was recorded).
 __global__ void gradient2d_kernel(const double *tt, const uint8_t *sgn, int NZ, int NX, double dz,
@@ -405,3 +440,128 @@ extern "C" int fteik2d_gradient(const double *tt, const uint8_t *sgn, int nz, in
                        (hipStream_t)stream, tt, sgn, nz + 1, nx + 1, dz, dx, P, vzero, ttgrad);
     return hip_status(hipGetLastError());
 }
+
+// ---------------------------------------------------------------------------------------
+// Batch of independent 2D models (fteik_solve_batch, batch.hip)
+// ---------------------------------------------------------------------------------------
+
+#include "batch_internal.hpp"
+
+namespace fteik {
+
+struct Batch2 {
+    std::vector<Prob2> probs;
+    std::vector<SrcParam> prm;   // all sources, model after model
+    std::vector<int> cum;        // [n2 + 1]
+    std::vector<const fteik_problem *> src;
+    char *base = nullptr;
+    size_t off_probs = 0, off_cum = 0, off_prm = 0;
+    int nsweep = 0, max_sets = 1;
+    bool grad = false;
+};
+
+namespace {
+void batch2_layout(const fteik_problem *p, int n, size_t *probs, size_t *cum, size_t *prm,
+                   size_t *total) {
+    int n2 = 0;
+    long long ns = 0;
+    for (int i = 0; i < n; ++i)
+        if (p[i].ndim == 2) {
+            ++n2;
+            ns += p[i].nsrc;
+        }
+    size_t off = 0;
+    *probs = off;
+    off += align256(sizeof(Prob2) * n2);
+    *cum = off;
+    off += align256(sizeof(int) * (n2 + 1));
+    *prm = off;
+    off += align256(sizeof(SrcParam) * ns);
+    *total = n2 ? off : 0;
+}
+}  // namespace
+
+size_t batch2_bytes(const fteik_problem *probs, int nprob) {
+    size_t a, b, c, t;
+    batch2_layout(probs, nprob, &a, &b, &c, &t);
+    return t;
+}
+
+int batch2_prepare(const fteik_problem *P, int n, int nsweep, int max_sets, int want_grad,
+                   char *ws, Batch2 **out, int *failed) {
+    Batch2 *b = new Batch2;
+    size_t total;
+    batch2_layout(P, n, &b->off_probs, &b->off_cum, &b->off_prm, &total);
+    b->base = ws;
+    b->nsweep = nsweep;
+    b->max_sets = max_sets;
+    b->grad = want_grad != 0;
+    b->cum.push_back(0);
+    SrcParam *prm_dev = (SrcParam *)(ws + b->off_prm);
+    for (int i = 0; i < n; ++i) {
+        const fteik_problem &q = P[i];
+        if (q.ndim != 2) continue;
+        if (q.nz < 1 || q.nx < 1 || q.nsrc < 1 || !q.slow || !q.src || !q.tt || !q.vzero ||
+            !q.sets_done || !(q.dz > 0.0) || !(q.dx > 0.0) || (want_grad && !q.sgn) ||
+            (long long)b->cum.back() + q.nsrc >= (1LL << 31)) {
+            delete b;
+            return FTEIK_ERR_ARG;
+        }
+        const size_t first = b->prm.size();
+        for (int s = 0; s < q.nsrc; ++s) {
+            SrcParam sp;
+            if (!place_source2(q.src + 2 * s, q.nz, q.nx, q.dz, q.dx, &sp)) {
+                if (failed) *failed = i;
+                delete b;
+                return FTEIK_ERR_SOURCE_OOB;
+            }
+            b->prm.push_back(sp);
+        }
+        Prob2 p2;
+        p2.slow = q.slow;
+        p2.nz = q.nz;
+        p2.nx = q.nx;
+        p2.dz = q.dz;
+        p2.dx = q.dx;
+        p2.prm = prm_dev + first;
+        p2.tt = q.tt;
+        p2.sgn = want_grad ? q.sgn : nullptr;
+        p2.vzero = q.vzero;
+        p2.sets_done = q.sets_done;
+        b->probs.push_back(p2);
+        b->cum.push_back(b->cum.back() + q.nsrc);
+        b->src.push_back(&q);
+    }
+    *out = b;
+    return 0;
+}
+
+void batch2_free(Batch2 *b) { delete b; }
+
+int batch2_enqueue(Batch2 *b, hipStream_t stream) {
+    const int n2 = (int)b->probs.size();
+    if (n2 == 0) return 0;
+    FTEIK_CHECK(hipMemcpyAsync(b->base + b->off_probs, b->probs.data(), sizeof(Prob2) * n2,
+                               hipMemcpyHostToDevice, stream));
+    FTEIK_CHECK(hipMemcpyAsync(b->base + b->off_cum, b->cum.data(), sizeof(int) * (n2 + 1),
+                               hipMemcpyHostToDevice, stream));
+    FTEIK_CHECK(hipMemcpyAsync(b->base + b->off_prm, b->prm.data(),
+                               sizeof(SrcParam) * b->prm.size(), hipMemcpyHostToDevice, stream));
+    // the host arrays are freed by the caller after this returns: finish the copies
+    FTEIK_CHECK(hipStreamSynchronize(stream));
+    for (int m = 0; m < n2; ++m) {
+        const fteik_problem &q = *b->src[m];
+        const long long total = (long long)(q.nz + 1) * (q.nx + 1) * q.nsrc;
+        const int blocks = (int)std::min<long long>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(fill2d_kernel, dim3(blocks), dim3(256), 0, stream, q.tt, total,
+                           b->grad ? q.sgn : (uint8_t *)nullptr);
+        FTEIK_CHECK(hipGetLastError());
+    }
+    const int lanes = b->cum.back();
+    hipLaunchKernelGGL(fteik2d_batch_kernel, dim3((lanes + 63) / 64), dim3(64), 0, stream,
+                       (const Prob2 *)(b->base + b->off_probs),
+                       (const int *)(b->base + b->off_cum), n2, b->nsweep, b->max_sets);
+    return hip_status(hipGetLastError());
+}
+
+}  // namespace fteik

@@ -82,6 +82,7 @@ struct SweepArgs {
     int tile_a;             // z extent of a tile / brick
     int nba, nbb, nbc;      // tile counts along z, x, y (memory grid, nodes 0..N-1)
     int ntiles, ndiag;
+    int gmax;               // source groups (workgroups per brick)
     int pass;               // diagonal schedule: pass of this launch
     int diag;               // diagonal schedule: brick anti-diagonal of this launch
     unsigned *log_node;     // logged solve: changed node (source * nodes + node) ...
@@ -424,6 +425,30 @@ __global__ __launch_bounds__(64 * W) void sweep3d_diag_kernel(const SweepArgs A)
     brick<GRAD, LOG, W>(A, A.pass, BA, BB, BC, g0, nact);
 }
 
+// Batch of models: one launch per (pass, anti-diagonal d) covers the bricks of d of every
+// model.  cum[m] = first block of model m (non-decreasing, cum[nmodel] = grid size); a
+// model's blocks enumerate (brick on d, source group), source group fastest.
+template <bool GRAD, int W>
+__global__ __launch_bounds__(64 * W) void sweep3d_multi_kernel(const SweepArgs *models,
+                                                               const int *cum, int nmodel,
+                                                               int pass, int diag) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = nmodel - 1;  // the last model whose first block is <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cum[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const SweepArgs A = models[lo];
+    const int local = b - cum[lo];
+    const int t = local / A.gmax, g = local - t * A.gmax;
+    const int nact = *A.nlist;
+    if (g * W >= nact) return;
+    int BA, BB, BC;
+    decode_brick(diag, t, A.nba, A.nbb, A.nbc, &BA, &BB, &BC);
+    brick<GRAD, false, W>(A, pass, BA, BB, BC, g * W, nact);
+}
+
 // Logged solve: closes the current pass's segment of the history.
 __global__ void log_mark_kernel(HistHeader *hdr, unsigned long long *seg,
                                 const unsigned long long *count) {
@@ -579,8 +604,8 @@ __global__ void seed3d_kernel(const double *slow, int nz, int nx, int ny, double
 // After each set: count it for every active source, retire sources whose set changed
 // nothing (further sets would repeat it exactly), clear the change flags and rebuild the
 // compacted list of sources the next set sweeps.  One block.
-__global__ void set_bookkeeping_kernel(int nsrc, int *sets_done, int *changed, int *active,
-                                       int *list, int *nlist) {
+__device__ void bookkeeping(int nsrc, int *sets_done, int *changed, int *active, int *list,
+                            int *nlist) {
     for (int s = threadIdx.x; s < nsrc; s += blockDim.x) {
         if (active[s]) {
             sets_done[s] += 1;
@@ -595,6 +620,21 @@ __global__ void set_bookkeeping_kernel(int nsrc, int *sets_done, int *changed, i
             if (active[s]) list[n++] = s;
         *nlist = n;
     }
+}
+
+__global__ void set_bookkeeping_kernel(int nsrc, int *sets_done, int *changed, int *active,
+                                       int *list, int *nlist) {
+    bookkeeping(nsrc, sets_done, changed, active, list, nlist);
+}
+
+// Per-model bookkeeping of a batch: one block per model.
+struct BookArgs {
+    int nsrc;
+    int *sets_done, *changed, *active, *list, *nlist;
+};
+__global__ void set_bookkeeping_multi_kernel(const BookArgs *b) {
+    const BookArgs B = b[blockIdx.x];
+    bookkeeping(B.nsrc, B.sets_done, B.changed, B.active, B.list, B.nlist);
 }
 
 // Gradient post-pass, _fteik3d.py:464-488: components with a recorded upwind sign are
@@ -773,6 +813,64 @@ size_t ws_bytes(int nz, int nx, int ny, int nsrc, Ws3 *w, char *base) {
     return off;
 }
 
+// brick height default: 32-plane bricks for batches (shorter bricks skip more of the sparse
+// later sets, taller ones waste less on the 14-step skew), 8 planes for a few sources
+// (latency bound).  Measured on MI355X.
+int default_tile(int nsrc) { return nsrc >= 32 ? 32 : 8; }
+
+// Sweep arguments of one model and its workspace carve-up (the history fields are
+// cleared; the logged solve sets them).
+void make_args(const double *slow, int nz, int nx, int ny, double dz, double dx, double dy,
+               int nsrc, int tile_a, int group, uint8_t *sgn, double *tt, int *sets_done,
+               char *workspace, SweepArgs *out, Ws3 *wout) {
+    const int NZ = nz + 1, NX = nx + 1, NY = ny + 1;
+    const long long nn = (long long)NZ * NX * NY;
+    Ws3 w;
+    ws_bytes(nz, nx, ny, nsrc, &w, workspace);
+    SweepArgs a;
+    a.slow = slow;
+    a.tt = tt;
+    a.sgn = sgn;
+    a.chg = w.chg;
+    a.tsum = w.tsum;
+    a.changed = w.changed;
+    a.list = w.list;
+    a.nlist = w.nlist;
+    a.sets_done = sets_done;
+    a.progress = w.progress;
+    a.queue = w.queue;
+    a.error = w.error;
+    a.cum_nb = w.cum_nb;
+    a.node_stride = nn;
+    a.nz = nz; a.nx = nx; a.ny = ny;
+    a.NZ = NZ; a.NX = NX; a.NY = NY;
+    if (tile_a <= 0) tile_a = default_tile(nsrc);
+    a.tile_a = std::min(tile_a, NZ);
+    a.nba = ceil_div(NZ, a.tile_a);
+    a.nbb = ceil_div(NX, kLanesB);
+    a.nbc = ceil_div(NY, kLanesC);
+    a.ntiles = a.nba * a.nbb * a.nbc;
+    a.ndiag = a.nba + a.nbb + a.nbc - 2;
+    a.pass = 0;
+    a.diag = 0;
+    a.log_node = nullptr;
+    a.log_old = nullptr;
+    a.log_count = nullptr;
+    a.log_cap = 0;
+    a.wait_ticks = 0;
+    a.gmax = ceil_div(nsrc, group);
+    a.dz = dz; a.dx = dx; a.dy = dy;
+    a.dz2i = 1.0 / dz / dz;
+    a.dx2i = 1.0 / dx / dx;
+    a.dy2i = 1.0 / dy / dy;
+    a.dz2dx2 = a.dz2i * a.dx2i;
+    a.dz2dy2 = a.dz2i * a.dy2i;
+    a.dx2dy2 = a.dx2i * a.dy2i;
+    a.dsum = a.dz2i + a.dx2i + a.dy2i;
+    *out = a;
+    *wout = w;
+}
+
 }  // namespace
 }  // namespace fteik
 
@@ -811,7 +909,9 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
     if (!(dz > 0.0) || !(dx > 0.0) || !(dy > 0.0)) return FTEIK_ERR_ARG;
     if (workspace_bytes < fteik3d_workspace_size(nz, nx, ny, nsrc)) return FTEIK_ERR_WORKSPACE;
     if (nsweep <= 0 && max_sets < 1) return FTEIK_ERR_ARG;
-    if (group == 0) group = nsrc >= 64 ? 4 : (nsrc >= 8 ? 2 : 1);
+    // defaults measured on MI355X (160x320x320, 256 sources): 2 sources per workgroup
+    // beat 4 and 8 (a wider workgroup waits at every step for its slowest source)
+    if (group == 0) group = nsrc >= 8 ? 2 : 1;
     if (group != 1 && group != 2 && group != 4 && group != 8) return FTEIK_ERR_ARG;
     if (schedule != 0 && schedule != 1) return FTEIK_ERR_ARG;
     // The persistent schedule stalled on its first GPU run (dependency waits on per-tile
@@ -828,52 +928,19 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
     const int NZ = nz + 1, NX = nx + 1, NY = ny + 1;
     const long long nn = (long long)NZ * NX * NY;
     Ws3 w;
-    ws_bytes(nz, nx, ny, nsrc, &w, (char *)workspace);
-
     SweepArgs a;
-    a.slow = slow;
-    a.tt = tt;
-    a.sgn = want_grad ? sgn : nullptr;
-    a.chg = w.chg;
-    a.tsum = w.tsum;
-    a.changed = w.changed;
-    a.list = w.list;
-    a.nlist = w.nlist;
-    a.sets_done = sets_done;
-    a.progress = w.progress;
-    a.queue = w.queue;
-    a.error = w.error;
-    a.cum_nb = w.cum_nb;
-    a.node_stride = nn;
-    a.nz = nz; a.nx = nx; a.ny = ny;
-    a.NZ = NZ; a.NX = NX; a.NY = NY;
-    if (tile_a <= 0) tile_a = nsrc >= 32 ? NZ : std::min(NZ, 16);
-    a.tile_a = std::min(tile_a, NZ);
-    a.nba = ceil_div(NZ, a.tile_a);
-    a.nbb = ceil_div(NX, kLanesB);
-    a.nbc = ceil_div(NY, kLanesC);
-    a.ntiles = a.nba * a.nbb * a.nbc;
-    a.ndiag = a.nba + a.nbb + a.nbc - 2;
-    a.pass = 0;
-    a.diag = 0;
+    make_args(slow, nz, nx, ny, dz, dx, dy, nsrc, tile_a, group, want_grad ? sgn : nullptr, tt,
+              sets_done, (char *)workspace, &a, &w);
     a.log_node = hv ? hv->node : nullptr;
     a.log_old = hv ? hv->old : nullptr;
     a.log_count = hv ? &hv->hdr->count : nullptr;
     a.log_cap = hv ? hcap : 0;
-    a.dz = dz; a.dx = dx; a.dy = dy;
-    a.dz2i = 1.0 / dz / dz;
-    a.dx2i = 1.0 / dx / dx;
-    a.dy2i = 1.0 / dy / dy;
-    a.dz2dx2 = a.dz2i * a.dx2i;
-    a.dz2dy2 = a.dz2i * a.dy2i;
-    a.dx2dy2 = a.dx2i * a.dy2i;
-    a.dsum = a.dz2i + a.dx2i + a.dy2i;
     std::vector<int> nb(a.ndiag), cum(a.ndiag + 1, 0);
     for (int d = 0; d < a.ndiag; ++d) {
         nb[d] = bricks_on_diag(d, a.nba, a.nbb, a.nbc);
         cum[d + 1] = cum[d] + nb[d];
     }
-    const long long gmax = ceil_div(nsrc, group);
+    const long long gmax = a.gmax;
     if (8LL * a.ntiles * gmax >= (1LL << 31)) return FTEIK_ERR_ARG;  // task index is an int
     const int pblocks = schedule == 0 ? pers_blocks(want_grad != 0, group) : 0;
     if (schedule == 0 && pblocks <= 0) return FTEIK_ERR_ARG;
@@ -1043,3 +1110,247 @@ extern "C" int fteik3d_gradient(const double *tt, const uint8_t *sgn, int nz, in
                        vzero, ttgrad);
     return hip_status(hipGetLastError());
 }
+
+// ---------------------------------------------------------------------------------------
+// Batch of independent 3D models (fteik_solve_batch, batch.hip)
+// ---------------------------------------------------------------------------------------
+
+#include "batch_internal.hpp"
+
+namespace fteik {
+
+namespace {
+
+template <bool G, int W>
+void launch_multi_t(unsigned blocks, hipStream_t st, const SweepArgs *models, const int *cum,
+                    int nmodel, int pass, int diag) {
+    hipLaunchKernelGGL((sweep3d_multi_kernel<G, W>), dim3(blocks), dim3(64 * W), 0, st, models,
+                       cum, nmodel, pass, diag);
+}
+
+// Device layout of a batch's 3D part: model arguments, bookkeeping arguments, the active
+// source count of every model (one int each, read back together), the block offsets of
+// every (anti-diagonal, model), then each model's own solve workspace.
+struct Batch3Layout {
+    size_t args, book, nlist, cum, models, total;
+    std::vector<size_t> model_off;
+    int dmax;
+};
+
+void batch3_layout(const fteik_problem *p, int n, Batch3Layout *L) {
+    int n3 = 0, dmax = 0;
+    L->model_off.clear();
+    size_t off = 0;
+    std::vector<int> idx;
+    for (int i = 0; i < n; ++i)
+        if (p[i].ndim == 3) {
+            idx.push_back(i);
+            ++n3;
+            const int NZ = p[i].nz + 1;
+            const int tile = std::min(p[i].tile_a > 0 ? p[i].tile_a : default_tile(p[i].nsrc), NZ);
+            dmax = std::max(dmax, ceil_div(NZ, tile) + ceil_div(p[i].nx + 1, kLanesB) +
+                                      ceil_div(p[i].ny + 1, kLanesC) - 2);
+        }
+    L->dmax = dmax;
+    L->args = off;
+    off += align256(sizeof(SweepArgs) * n3);
+    L->book = off;
+    off += align256(sizeof(BookArgs) * n3);
+    L->nlist = off;
+    off += align256(sizeof(int) * n3);
+    L->cum = off;
+    off += align256(sizeof(int) * (size_t)dmax * (n3 + 1));
+    L->models = off;
+    for (int i : idx) {
+        L->model_off.push_back(off);
+        off += align256(ws_bytes(p[i].nz, p[i].nx, p[i].ny, p[i].nsrc, nullptr, nullptr));
+    }
+    L->total = n3 ? off : 0;
+}
+
+}  // namespace
+
+struct Batch3 {
+    std::vector<SweepArgs> args;
+    std::vector<BookArgs> book;
+    std::vector<std::vector<SrcParam>> prm;
+    std::vector<Ws3> ws;
+    std::vector<int> cum;          // [dmax][n3 + 1]
+    std::vector<const fteik_problem *> probs;
+    Batch3Layout L;
+    char *base = nullptr;
+    int nsweep = 0, max_sets = 1, group = 2;
+    bool grad = false;
+};
+
+size_t batch3_bytes(const fteik_problem *probs, int nprob) {
+    Batch3Layout L;
+    batch3_layout(probs, nprob, &L);
+    return L.total;
+}
+
+int batch3_prepare(const fteik_problem *P, int n, int nsweep, int max_sets, int want_grad,
+                   int group, char *ws, Batch3 **out, int *failed) {
+    if (group == 0) group = 2;
+    if (group != 1 && group != 2 && group != 4 && group != 8) return FTEIK_ERR_ARG;
+    Batch3 *b = new Batch3;
+    batch3_layout(P, n, &b->L);
+    b->base = ws;
+    b->nsweep = nsweep;
+    b->max_sets = max_sets;
+    b->group = group;
+    b->grad = want_grad != 0;
+    int *nlist_dev = (int *)(ws + b->L.nlist);
+    int k = 0;
+    for (int i = 0; i < n; ++i) {
+        const fteik_problem &q = P[i];
+        if (q.ndim != 3) continue;
+        if (q.nz < 1 || q.nx < 1 || q.ny < 1 || q.nsrc < 1 || !q.slow || !q.src || !q.tt ||
+            !q.vzero || !q.sets_done || !(q.dz > 0.0) || !(q.dx > 0.0) || !(q.dy > 0.0) ||
+            (want_grad && !q.sgn)) {
+            delete b;
+            return FTEIK_ERR_ARG;
+        }
+        std::vector<SrcParam> prm(q.nsrc);
+        for (int s = 0; s < q.nsrc; ++s)
+            if (!place_source3(q.src + 3 * s, q.nz, q.nx, q.ny, q.dz, q.dx, q.dy, &prm[s])) {
+                if (failed) *failed = i;
+                delete b;
+                return FTEIK_ERR_SOURCE_OOB;
+            }
+        SweepArgs a;
+        Ws3 w;
+        make_args(q.slow, q.nz, q.nx, q.ny, q.dz, q.dx, q.dy, q.nsrc, q.tile_a, group,
+                  want_grad ? q.sgn : nullptr, q.tt, q.sets_done, ws + b->L.model_off[k], &a, &w);
+        a.nlist = w.nlist = nlist_dev + k;
+        if ((long long)a.ntiles * a.gmax >= (1LL << 31)) {
+            delete b;
+            return FTEIK_ERR_ARG;
+        }
+        BookArgs bk;
+        bk.nsrc = q.nsrc;
+        bk.sets_done = q.sets_done;
+        bk.changed = w.changed;
+        bk.active = w.active;
+        bk.list = w.list;
+        bk.nlist = w.nlist;
+        b->args.push_back(a);
+        b->book.push_back(bk);
+        b->prm.push_back(std::move(prm));
+        b->ws.push_back(w);
+        b->probs.push_back(&q);
+        ++k;
+    }
+    const int n3 = k;
+    b->cum.assign((size_t)b->L.dmax * (n3 + 1), 0);
+    for (int d = 0; d < b->L.dmax; ++d) {
+        long long c = 0;
+        for (int m = 0; m < n3; ++m) {
+            b->cum[(size_t)d * (n3 + 1) + m] = (int)c;
+            const SweepArgs &a = b->args[m];
+            if (d < a.ndiag) c += (long long)bricks_on_diag(d, a.nba, a.nbb, a.nbc) * a.gmax;
+        }
+        if (c >= (1LL << 31)) {
+            delete b;
+            return FTEIK_ERR_ARG;
+        }
+        b->cum[(size_t)d * (n3 + 1) + n3] = (int)c;
+    }
+    *out = b;
+    return 0;
+}
+
+void batch3_free(Batch3 *b) { delete b; }
+
+int batch3_run(Batch3 *b, int use_graph, hipStream_t stream) {
+    const int n3 = (int)b->args.size();
+    if (n3 == 0) return 0;
+    char *ws = b->base;
+    SweepArgs *args_dev = (SweepArgs *)(ws + b->L.args);
+    BookArgs *book_dev = (BookArgs *)(ws + b->L.book);
+    int *nlist_dev = (int *)(ws + b->L.nlist);
+    int *cum_dev = (int *)(ws + b->L.cum);
+    FTEIK_CHECK(hipMemcpyAsync(args_dev, b->args.data(), sizeof(SweepArgs) * n3,
+                               hipMemcpyHostToDevice, stream));
+    FTEIK_CHECK(hipMemcpyAsync(book_dev, b->book.data(), sizeof(BookArgs) * n3,
+                               hipMemcpyHostToDevice, stream));
+    FTEIK_CHECK(hipMemcpyAsync(cum_dev, b->cum.data(), sizeof(int) * b->cum.size(),
+                               hipMemcpyHostToDevice, stream));
+    for (int m = 0; m < n3; ++m)
+        FTEIK_CHECK(hipMemcpyAsync(b->ws[m].prm, b->prm[m].data(),
+                                   sizeof(SrcParam) * b->prm[m].size(), hipMemcpyHostToDevice,
+                                   stream));
+    // the host arrays stay alive in *b; finish the copies before the first kernel anyway so
+    // that a failed launch leaves nothing pending on them
+    FTEIK_CHECK(hipStreamSynchronize(stream));
+    for (int m = 0; m < n3; ++m) {
+        const fteik_problem &q = *b->probs[m];
+        const SweepArgs &a = b->args[m];
+        const long long total = a.node_stride * q.nsrc;
+        const int blocks = (int)std::min<long long>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, stream, q.tt, total,
+                           b->grad ? q.sgn : (uint8_t *)nullptr, a.chg);
+        FTEIK_CHECK(hipGetLastError());
+        const long long nt = (long long)a.ntiles * q.nsrc;
+        hipLaunchKernelGGL(zero_bytes_kernel,
+                           dim3((unsigned)std::min<long long>((nt + 255) / 256, 65536)), dim3(256),
+                           0, stream, a.tsum, nt);
+        FTEIK_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(seed3d_kernel, dim3((q.nsrc * 8 + 255) / 256), dim3(256), 0, stream,
+                           q.slow, q.nz, q.nx, q.ny, q.dz, q.dx, q.dy, b->ws[m].prm, q.nsrc, q.tt,
+                           a.node_stride, q.vzero, q.sets_done, a.changed, b->ws[m].active,
+                           b->ws[m].list, b->ws[m].nlist);
+        FTEIK_CHECK(hipGetLastError());
+    }
+
+    const int dmax = b->L.dmax;
+    auto enqueue_set = [&]() -> int {
+        for (int p = 0; p < 8; ++p)
+            for (int d = 0; d < dmax; ++d) {
+                const int blocks = b->cum[(size_t)d * (n3 + 1) + n3];
+                if (blocks == 0) continue;
+                FTEIK_DISPATCH_W(launch_multi_t, b->grad, b->group, (unsigned)blocks, stream,
+                                 args_dev, cum_dev + (size_t)d * (n3 + 1), n3, p, d);
+            }
+        hipLaunchKernelGGL(set_bookkeeping_multi_kernel, dim3(n3), dim3(256), 0, stream,
+                           book_dev);
+        return hip_status(hipGetLastError());
+    };
+
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    if (use_graph && stream) {
+        FTEIK_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int st = enqueue_set();
+        hipError_t e = hipStreamEndCapture(stream, &graph);
+        if (st) return st;
+        FTEIK_CHECK(e);
+        FTEIK_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    auto run_set = [&]() -> int {
+        if (exec) return hip_status(hipGraphLaunch(exec, stream));
+        return enqueue_set();
+    };
+    int status = 0;
+    if (b->nsweep > 0) {
+        for (int s = 0; s < b->nsweep && !status; ++s) status = run_set();
+    } else {
+        std::vector<int> nact(n3);
+        for (int s = 0; s < b->max_sets && !status; ++s) {
+            status = run_set();
+            if (status) break;
+            hipError_t e = hipMemcpyAsync(nact.data(), nlist_dev, sizeof(int) * n3,
+                                          hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            status = hip_status(e);
+            bool any = false;
+            for (int m = 0; m < n3; ++m) any |= nact[m] > 0;
+            if (!any) break;
+        }
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    return status;
+}
+
+}  // namespace fteik

@@ -5,6 +5,8 @@ workloads (numpy only, so every machine regenerates them bit for bit).
 - ``model_3d_layers``: 3D layered medium with a smooth random perturbation (200^3, 5 m)
 - ``model_3d_inclusions``: 3D linear-gradient medium with 20 spherical inclusions
   (160 x 320 x 320, 12.5 m) with a 16 x 16 surface source grid and 64 x 64 receivers
+- ``model_3d_smooth_slowness``: gradient workload (128 x 256 x 256, 10 m)
+- ``uneven_batch``: 16 2D and 32 3D independent models of uneven size (1-32 sources each)
 """
 import numpy as np
 
@@ -103,3 +105,46 @@ def model_3d_smooth_slowness(seed=1005, shape=(128, 256, 256), d=10.0, sigma=4.0
     ry = (np.arange(rec_grid[1]) + 0.5) * ey / rec_grid[1]
     rec = np.stack(np.meshgrid(np.zeros(1), rx, ry, indexing="ij"), -1).reshape(-1, 3)
     return true, start, (d, d, d), src, rec
+
+
+def smooth_random_field(rng, shape, sigma):
+    """N(0,1) noise smoothed by a Gaussian of `sigma` cells (periodic, applied in Fourier
+    space: fast enough for 10^7-cell models), rescaled to [0, 1]."""
+    p = rng.standard_normal(shape)
+    f = np.fft.rfftn(p)
+    k2 = np.zeros(f.shape)
+    for ax, n in enumerate(shape):
+        k = np.fft.rfftfreq(n) if ax == len(shape) - 1 else np.fft.fftfreq(n)
+        sh = [1] * len(shape)
+        sh[ax] = len(k)
+        k2 = k2 + (2.0 * np.pi * k.reshape(sh)) ** 2
+    p = np.fft.irfftn(f * np.exp(-0.5 * sigma * sigma * k2), s=shape,
+                      axes=tuple(range(len(shape))))
+    return (p - p.min()) / (p.max() - p.min())
+
+
+def uneven_batch(seed=1004, n2=16, n3=32, side2=(64, 1024), side3=((48, 192), (48, 256)),
+                 d=10.0, nsrc=(1, 32), vel=(1500.0, 6000.0), sigma=(4.0, 12.0)):
+    """Uneven batch of independent models: n2 2D models with sides log-uniform in side2,
+    n3 3D models with nz log-uniform in side3[0] and nx, ny in side3[1]; velocities in
+    `vel` from smooth random fields (sigma uniform in `sigma` cells); nsrc[0]..nsrc[1]
+    sources per model, uniform inside it.  Returns a list of (velocity, gridsize, sources),
+    2D models first."""
+    rng = np.random.default_rng(seed)
+
+    def logu(lo, hi):
+        return int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+
+    out = []
+    for m in range(n2 + n3):
+        if m < n2:
+            shape = (logu(*side2), logu(*side2))
+        else:
+            shape = (logu(*side3[0]), logu(*side3[1]), logu(*side3[1]))
+        p = smooth_random_field(rng, shape, rng.uniform(*sigma))
+        v = vel[0] + (vel[1] - vel[0]) * p
+        ns = int(rng.integers(nsrc[0], nsrc[1] + 1))
+        ext = np.array(shape, dtype=np.float64) * d
+        src = rng.uniform(0.0, 1.0, size=(ns, len(shape))) * ext
+        out.append((v, (d,) * len(shape), src))
+    return out

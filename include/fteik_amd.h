@@ -190,6 +190,52 @@ int fteik2d_gradient(const double *tt, const uint8_t *sgn, int nz, int nx, doubl
                      const double *src, double vzero, double *ttgrad, void *stream);
 
 /* ---------------------------------------------------------------------------------- */
+/* Batch of independent models                                                          */
+/* ---------------------------------------------------------------------------------- */
+
+/* One model of a batch (fteik_solve_batch): the arguments of one fteik3d_solve or
+ * fteik2d_solve call.  ndim 2: ny and dy are ignored, src is [nsrc, 2] and the grids are
+ * [nsrc, nz+1, nx+1]. */
+typedef struct fteik_problem {
+    int ndim;             /* 2 or 3 */
+    int nz, nx, ny;       /* cells */
+    double dz, dx, dy;
+    const double *slow;   /* device [nz, nx(, ny)] slowness */
+    const double *src;    /* host [nsrc, ndim] sources relative to the model's origin */
+    int nsrc;
+    int tile_a;           /* 3D brick height along z (0 = library default for nsrc) */
+    double *tt;           /* device [nsrc, nz+1, nx+1(, ny+1)] traveltimes (output) */
+    uint8_t *sgn;         /* device upwind signs, shaped as tt (uint8); NULL without want_grad */
+    double *vzero;        /* device [nsrc] (output) */
+    int *sets_done;       /* device [nsrc] (output) */
+} fteik_problem;
+
+/* Device workspace (bytes) of fteik_solve_batch for these problems. */
+size_t fteik_batch_workspace_size(const fteik_problem *probs, int nprob);
+
+/*
+ * No single counterpart in the reference, which solves one model per Eikonal2D/3D.solve
+ * call (fteikpy/_solver.py:48-54, :120-126; solve2d / solve3d, _fteik2d.py:667-674,
+ * _fteik3d.py:516-525): solves many independent models of uneven size in one call.  Each
+ * problem's outputs equal those of its own fteik3d_solve / fteik2d_solve call bit for bit.
+ * The 3D models share every sweep launch -- one launch per (pass, brick anti-diagonal)
+ * covers the bricks of all models and their active sources -- and converge per source;
+ * the 2D models (one lane per source, the reference's serial order) run concurrently on a
+ * library-owned stream joined back into `stream`.
+ *
+ * nsweep, max_sets  as fteik3d_solve (nsweep <= 0: converge, at most max_sets sets)
+ * want_grad         record upwind signs: every problem must then pass `sgn`
+ * group             3D sources per workgroup (0 = default; 1, 2, 4, 8), the same for all
+ * failed            host out (may be NULL): index of the first problem with a source out
+ *                   of bound, else -1
+ * Checks every source of every problem before enqueuing anything (returns 1, "source out
+ * of bound").  In converge mode the call synchronizes `stream` once per set.
+ */
+int fteik_solve_batch(const fteik_problem *probs, int nprob, int nsweep, int max_sets,
+                      int want_grad, int group, void *workspace, size_t workspace_bytes,
+                      int use_graph, int *failed, void *stream);
+
+/* ---------------------------------------------------------------------------------- */
 /* Interpolation                                                                       */
 /* ---------------------------------------------------------------------------------- */
 

@@ -156,3 +156,75 @@ def test_history_size_round_trip():
     for n in (1, 16, 17, 1000, 123457):
         for p in (8, 400):
             assert L.fteik3d_history_size(n, p) < L.fteik3d_history_size(n + 64, p)
+
+
+def _problem(ndim, shape, src, fake):
+    from fteikpy_amd import _lib
+
+    p = _lib.Problem()
+    p.ndim = ndim
+    p.nz, p.nx = shape[0], shape[1]
+    p.ny = shape[2] if ndim == 3 else 0
+    p.dz = p.dx = p.dy = 1.0
+    p.slow = p.tt = p.vzero = p.sets_done = fake
+    p.sgn = None
+    p.src = src.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    p.nsrc = len(src)
+    p.tile_a = 0
+    return p
+
+
+def test_batch_checks_every_source_before_device_work():
+    """fteik_solve_batch checks all problems on the host first: an out-of-bound source in
+    any model returns code 1 with that model's index, and nothing touches the fake device
+    pointers."""
+    from fteikpy_amd import _lib
+
+    L = _lib.lib()
+    fake = 0x1000
+    good3 = np.array([[1.0, 2.0, 3.0], [4.0, 0.5, 0.5]])
+    good2 = np.array([[1.0, 2.0]])
+    bad2 = np.array([[1.0, 2.0], [0.0, 9.5]])
+    arr = (_lib.Problem * 3)(_problem(3, (5, 6, 7), good3, fake),
+                             _problem(2, (8, 9), good2, fake),
+                             _problem(2, (8, 9), bad2, fake))
+    wsb = L.fteik_batch_workspace_size(arr, 3)
+    assert wsb > 0
+    failed = ctypes.c_int(-7)
+    st = L.fteik_solve_batch(arr, 3, 0, 10, 0, 0, ctypes.c_void_p(fake), wsb, 0,
+                             ctypes.byref(failed), None)
+    assert st == _lib.ERR_SOURCE_OOB and failed.value == 2
+    # a 3D source out of bound is found as well
+    bad3 = np.array([[1.0, 2.0, 7.5]])
+    arr[0] = _problem(3, (5, 6, 7), bad3, fake)
+    arr[2] = _problem(2, (8, 9), good2, fake)
+    st = L.fteik_solve_batch(arr, 3, 0, 10, 0, 0, ctypes.c_void_p(fake), wsb, 0,
+                             ctypes.byref(failed), None)
+    assert st == _lib.ERR_SOURCE_OOB and failed.value == 0
+    # bad ndim, too small a workspace, gradients without sign buffers
+    arr[0] = _problem(4, (5, 6, 7), good3, fake)
+    assert L.fteik_solve_batch(arr, 3, 0, 10, 0, 0, ctypes.c_void_p(fake), wsb, 0, None,
+                               None) == _lib.ERR_ARG
+    arr[0] = _problem(3, (5, 6, 7), good3, fake)
+    assert L.fteik_solve_batch(arr, 3, 0, 10, 0, 0, ctypes.c_void_p(fake), wsb - 1, 0, None,
+                               None) == _lib.ERR_WORKSPACE
+    assert L.fteik_solve_batch(arr, 3, 0, 10, 1, 0, ctypes.c_void_p(fake), wsb, 0, None,
+                               None) == _lib.ERR_ARG
+    assert L.fteik_solve_batch(arr, 3, 0, 10, 0, 3, ctypes.c_void_p(fake), wsb, 0, None,
+                               None) == _lib.ERR_ARG
+
+
+def test_autotune_records_and_persists(tmp_path):
+    from fteikpy_amd import autotune
+
+    autotune.clear()
+    assert autotune.lookup3d((10, 20, 30), 4) == (0, 0)
+    autotune.record3d((10, 20, 30), 4, 16, 2)
+    assert autotune.lookup3d((10, 20, 30), 4) == (16, 2)
+    assert autotune.lookup3d((10, 20, 30), 5) == (0, 0)
+    path = str(tmp_path / "tune.json")
+    autotune.save(path)
+    autotune.clear()
+    autotune.load(path)
+    assert autotune.lookup3d((10, 20, 30), 4) == (16, 2)
+    autotune.clear()
