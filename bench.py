@@ -26,6 +26,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_NSU = 24  # FP64 tt read + tt write + slowness read per node update
 
+# HBM bytes per launch of the dominant kernel, sweep3d_diag_kernel<false, false, 2>, on the
+# default workload (160 x 320 x 320, 256 sources, 32-plane bricks, 2 sources per
+# workgroup): rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script, averaged
+# over the 26832 launches of one converged solve and scaled by a 1 GiB device copy
+# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Commands and figures:
+# profiles/sweep3d_traffic__inclusions_160x320x320_256src.md
+PMC_TRAFFIC = {"grid": (160, 320, 320), "sources": 256, "tile_a": 32, "group": 2,
+               "fetch_bytes": 204775.71 * 2.0 * 1024, "write_bytes": 8369.75 * 1.0 * 1024}
+
 
 def main():
     ap = argparse.ArgumentParser()
@@ -46,7 +55,13 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=3,
                     help="timed forward+adjoint iterations of the gradient workload (0 = skip)")
     ap.add_argument("--adj-shape", type=str, default="128,256,256")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only copy GIB GiB once on the device (torch clone) and exit: the "
+                         "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); sources are independent, "
@@ -75,6 +90,9 @@ def main():
             # brick height and group for this workload, timed on it before the warm-up
             tune = fteikpy_amd.autotune.tune3d(eik, src, tiles=(16, 32, 48, 0),
                                                use_graph=use_graph)
+        if tune:
+            log(f"autotune: tile_a {tune['tile_a']} group {tune['group']} "
+                f"({tune['seconds']:.1f} s)")
         out = torch.empty((nsrc, nz + 1, nx + 1, ny + 1), dtype=torch.float64, device=dev)
         for _ in range(args.warmup):
             eik.solve_tensors(src, nsweep=None, tile_a=args.tile, use_graph=use_graph, out=out)
@@ -92,22 +110,27 @@ def main():
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         gpu_s = ev0.elapsed_time(ev1) / 1e3
+        log(f"batched: {args.steps} converged solves in {wall:.2f} s")
         batched_tt0 = tt[0].cpu().numpy()
         k_sets = sets.copy()
 
         # plain: one source at a time
         nplain = min(args.plain, nsrc)
-        eik.solve_tensors(src[:1], nsweep=None, tile_a=args.plain_tile)  # warm
-        torch.cuda.synchronize()
-        tp = time.perf_counter()
-        plain_tt0 = None
-        for i in range(nplain):
-            ttp, _, _, _ = eik.solve_tensors(src[i:i + 1], nsweep=None, tile_a=args.plain_tile)
-            if i == 0:
-                plain_tt0 = ttp[0]
-        torch.cuda.synchronize()
-        plain_s = time.perf_counter() - tp
-        plain_equal = bool(np.array_equal(plain_tt0.cpu().numpy(), batched_tt0))
+        plain_s, plain_equal = None, None
+        if nplain > 0:
+            eik.solve_tensors(src[:1], nsweep=None, tile_a=args.plain_tile)  # warm
+            torch.cuda.synchronize()
+            tp = time.perf_counter()
+            plain_tt0 = None
+            for i in range(nplain):
+                ttp, _, _, _ = eik.solve_tensors(src[i:i + 1], nsweep=None,
+                                                 tile_a=args.plain_tile)
+                if i == 0:
+                    plain_tt0 = ttp[0]
+            torch.cuda.synchronize()
+            plain_s = time.perf_counter() - tp
+            plain_equal = bool(np.array_equal(plain_tt0.cpu().numpy(), batched_tt0))
+            log(f"plain: {nplain} solves in {plain_s:.2f} s")
 
     del out, tt
     torch.cuda.empty_cache()
@@ -117,6 +140,17 @@ def main():
     alg_bytes = sets_total * 8 * nodes * BYTES_PER_NSU
     achieved = alg_bytes / gpu_s / 1e9
     value = nsrc * args.steps / wall
+    # launches of the sweep kernel in the timed region: every set of a batch launches one
+    # kernel per (pass, brick anti-diagonal), until its last source converged
+    tile = args.tile or (tune["tile_a"] if tune else (32 if nsrc >= 32 else 8))
+    group = tune["group"] if tune else (2 if nsrc >= 8 else 1)
+    tile = min(tile, nz + 1)
+    ndiag = -(-(nz + 1) // tile) + -(-(nx + 1) // 8) + -(-(ny + 1) // 8) - 2
+    launches = args.steps * int(k_sets.max()) * 8 * ndiag
+    traffic = None
+    if (tuple(shape) == PMC_TRAFFIC["grid"] and nsrc == PMC_TRAFFIC["sources"]
+            and tile == PMC_TRAFFIC["tile_a"] and group == PMC_TRAFFIC["group"]):
+        traffic = PMC_TRAFFIC["fetch_bytes"] + PMC_TRAFFIC["write_bytes"]
 
     cpu = None
     if not args.no_cpu:
@@ -128,6 +162,7 @@ def main():
         tc = time.perf_counter()
         oracle.solve3d(1.0 / v, *d, src[:ncs], nsweep=nsets, nthreads=ncores)
         tcpu = time.perf_counter() - tc
+        log(f"cpu baseline: {tcpu:.1f} s")
         # the CPU path evaluates every node in every set, so a converged grid costs its set
         # count (measured above for the same sources) times the per-set time
         mean_sets = float(k_sets[:ncs].mean())
@@ -159,18 +194,45 @@ def main():
                    "tile_a": args.tile or (tune["tile_a"] if tune else 0),
                    "group": tune["group"] if tune else 0},
         "autotune": tune,
-        "plain_value": nplain / plain_s,
+        "plain_value": nplain / plain_s if plain_s else None,
         "plain_sources_timed": nplain,
-        "batched_over_plain": value / (nplain / plain_s),
+        "batched_over_plain": value / (nplain / plain_s) if plain_s else None,
         "batched_equals_plain_bitwise": plain_equal,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "note": "24 B per node update x 8 passes x nodes x sets, over HIP-event "
-                             "time of the timed solves"},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": alg_bytes / launches,
+                     "launches": launches, "us_per_launch": gpu_s / launches * 1e6,
+                     "note": "achieved = algorithmic bytes per launch (24 B per node update x 8 "
+                             "passes x nodes x sets executed, over the launches of the timed "
+                             "solves) / mean launch time (HIP events on the kernels' stream); "
+                             "traffic = measured HBM bytes per launch (rocprofv3 FETCH_SIZE + "
+                             "WRITE_SIZE, copy-calibrated; null for other configurations): "
+                             "below the algorithmic bytes because inactive bricks and "
+                             "unchanged nodes are skipped"},
         "cpu_baseline": cpu,
         "fwd_adjoint": adj,
     }
     print(json.dumps(res), flush=True)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def copy_calibration(gib):
+    """One device copy of `gib` GiB (read once, written once): the reference kernel for the
+    HBM byte counters."""
+    import torch
+
+    n = int(gib * (1 << 30)) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    y = x.clone()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "seconds": dt,
+                      "checksum": float(y[::4096].sum())}), flush=True)
 
 
 def fwd_adjoint(args, torch, fa, synthetic, dev):
@@ -217,6 +279,7 @@ def fwd_adjoint(args, torch, fa, synthetic, dev):
         torch.cuda.synchronize()
         fwd.append(time.perf_counter() - t0)
         del tt
+    log(f"fwd+adjoint: {args.adj_steps} iterations")
     s_iter = float(np.median(times))
     f = float(min(fwd))
     nodes = np.prod(np.array(shape) + 1) * len(src)

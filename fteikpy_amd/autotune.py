@@ -57,10 +57,17 @@ def load(path):
             _CHOICES[k] = (int(v[0]), int(v[1]))
 
 
+def default3d(nsrc):
+    """The library's defaults for nsrc sources (fteik3d_solve with tile_a = group = 0)."""
+    return (32 if nsrc >= 32 else 8), (2 if nsrc >= 8 else 1)
+
+
 def tune3d(eik, sources, *, tiles=TILES, groups=GROUPS, max_sets=100, use_graph=True,
-           repeats=1, record=True):
+           repeats=1, record=True, margin=0.02):
     """Time converged solves of `sources` on the Eikonal3D `eik` and record the fastest
-    (tile_a, group).  Returns {"tile_a", "group", "seconds", "timings": {(tile_a, group): s}}.
+    (tile_a, group); the library default is kept unless a candidate beats it by more than
+    `margin` (a fraction of its time).  Returns {"tile_a", "group", "seconds",
+    "timings": {"tile_a:group": s}}.
 
     Every candidate computes the same traveltimes bit for bit (the sweep order within a
     pass does not change the result); only the time differs.  The solves write into one
@@ -91,11 +98,16 @@ def tune3d(eik, sources, *, tiles=TILES, groups=GROUPS, max_sets=100, use_graph=
         timings[(tile, group)] = best
         return best
 
-    g0 = 2 if 2 in groups else groups[0]
+    dt, dg = default3d(n)
+    dt = min(dt, nz + 1)
+    g0 = dg if dg in groups else groups[0]
     run(tiles[0], g0)  # warm-up: first launches, graph machinery
     timings.clear()
     best_tile = min(tiles, key=lambda t: run(t, g0))
     best_group = min(groups, key=lambda g: run(best_tile, g))
+    if dt in tiles and dg in groups:
+        if run(dt, dg) <= timings[(best_tile, best_group)] * (1.0 + margin):
+            best_tile, best_group = dt, dg
     del out
     if record:
         record3d(eik.shape, n, best_tile, best_group)
