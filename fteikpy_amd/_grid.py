@@ -114,6 +114,65 @@ class BaseGrid:
             return self._dev.contiguous(), 1
         return self._dev, k
 
+    def resample(self, new_shape, method="linear"):
+        """Resample the grid on the device (fteikpy/_base.py:81-112, :165-201): values at
+        linspace(axis[0], axis[-1], new_shape[a]) per axis by 'linear' or 'nearest'
+        interpolation, as scipy's RegularGridInterpolator computes them (bit for bit)."""
+        torch = _torch()
+        if method not in ("linear", "nearest"):
+            raise ValueError(f"Method '{method}' is not defined")
+        nd = self._ndim
+        new_shape = tuple(int(n) for n in new_shape)
+        if len(new_shape) != nd:
+            raise ValueError(f"new_shape must have {nd} entries")
+        if min(self.shape) < 2:
+            raise ValueError("resampling needs at least 2 nodes per axis")
+        src = self._dev.contiguous()
+        out = torch.empty(new_shape, dtype=torch.float64, device=device())
+        shp = (ctypes.c_int * nd)(*self.shape)
+        nshp = (ctypes.c_int * nd)(*new_shape)
+        o, op = _lib.host_f8(self._origin)
+        d, dp = _lib.host_f8(self._gridsize)
+        L = _lib.lib()
+        wsb = L.fteik_resample_workspace_size(nd, nshp)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=device())
+        st = L.fteik_resample(_ptr(src), nd, shp, op, dp, nshp, 0 if method == "linear" else 1,
+                              _ptr(out), _ptr(ws), wsb, stream_ptr())
+        _lib.check(st)
+        self._dev = out
+        self._host = None
+        # as the reference computes it: after the grid is replaced, so self.shape is
+        # already new_shape (fteikpy/_base.py:110-112)
+        self._gridsize = tuple(a * b / c for a, b, c in zip(self.gridsize, self.shape, new_shape))
+
+    def smooth(self, sigma):
+        """Gaussian smoothing on the device (fteikpy/_base.py:114-123, :203-212): sigma in
+        meters (scalar or one per axis), scipy.ndimage.gaussian_filter's kernel (truncated at
+        4 sigma, normalised) and 'reflect' boundaries, bit for bit."""
+        torch = _torch()
+        nd = self._ndim
+        sigma = np.full(nd, sigma) if np.ndim(sigma) == 0 else np.asarray(sigma)
+        sig = sigma / np.asarray(self._gridsize)
+        cur = self._dev.contiguous()
+        shp = (ctypes.c_int * nd)(*self.shape)
+        L = _lib.lib()
+        for axis in range(nd):
+            sd = float(sig[axis])
+            if not sd > 1e-15:
+                continue
+            radius = int(4.0 * sd + 0.5)
+            x = np.arange(-radius, radius + 1)
+            phi = np.exp(-0.5 / (sd * sd) * x ** 2)
+            phi = phi / phi.sum()
+            w = torch.as_tensor(np.ascontiguousarray(phi[::-1]), device=device())
+            nxt = torch.empty_like(cur)
+            st = L.fteik_correlate1d(_ptr(cur), _ptr(nxt), nd, shp, axis, _ptr(w), radius,
+                                     stream_ptr())
+            _lib.check(st)
+            cur = nxt
+        self._dev = cur
+        self._host = None
+
     def _interp(self, points, fill_value):
         torch = _torch()
         nd = self._ndim

@@ -71,6 +71,11 @@ SIGNATURES = {
     "fteik_batch_workspace_size": (ctypes.c_size_t, [_pp, _c_int]),
     "fteik_solve_batch": (_c_int, [_pp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp,
                                    ctypes.c_size_t, _c_int, ctypes.POINTER(_c_int), _vp]),
+    "fteik_correlate1d": (_c_int, [_vp, _vp, _c_int, ctypes.POINTER(_c_int), _c_int, _vp, _c_int,
+                                   _vp]),
+    "fteik_resample_workspace_size": (ctypes.c_size_t, [_c_int, ctypes.POINTER(_c_int)]),
+    "fteik_resample": (_c_int, [_vp, _c_int, ctypes.POINTER(_c_int), _dp, _dp,
+                                ctypes.POINTER(_c_int), _c_int, _vp, _vp, ctypes.c_size_t, _vp]),
     "fteik2d_workspace_size": (ctypes.c_size_t, [_c_int]),
     "fteik2d_solve": (_c_int, [_vp, _c_int, _c_int, _c_dbl, _c_dbl, _dp, _c_int, _c_int, _c_int,
                                _c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),

@@ -236,6 +236,35 @@ int fteik_solve_batch(const fteik_problem *probs, int nprob, int nsweep, int max
                       int use_graph, int *failed, void *stream);
 
 /* ---------------------------------------------------------------------------------- */
+/* Grid preparation (smoothing, resampling)                                            */
+/* ---------------------------------------------------------------------------------- */
+
+/*
+ * One axis of BaseGrid2D/3D.smooth (fteikpy/_base.py:114-123, :203-212), which calls
+ * scipy.ndimage.gaussian_filter: correlation of `in` (device, C-ordered `shape`, ndim 1..3)
+ * along `axis` with the symmetric kernel `weights` (device, 2 radius + 1 values) and
+ * 'reflect' boundaries, into `out` (device, same shape, not aliasing `in`).  A Gaussian
+ * filter is one call per axis, axis 0 first, each reading the previous output.
+ */
+int fteik_correlate1d(const double *in, double *out, int ndim, const int *shape, int axis,
+                      const double *weights, int radius, void *stream);
+
+/* Device workspace (bytes) of fteik_resample. */
+size_t fteik_resample_workspace_size(int ndim, const int *new_shape);
+
+/*
+ * BaseGrid2D/3D.resample (fteikpy/_base.py:81-112, :165-201), which evaluates
+ * scipy.interpolate.RegularGridInterpolator (method 0 'linear', 1 'nearest') of the grid
+ * `in` (device, `shape` nodes at origin + gridsize * i) on the nodes of
+ * linspace(axis[0], axis[-1], new_shape[a]) per axis, into `out` (device, new_shape).
+ * origin, gridsize, shape, new_shape: host [ndim], ndim 2 or 3, every shape[a] >= 2.
+ * Synchronizes the stream once (upload of the per-axis cell indices).
+ */
+int fteik_resample(const double *in, int ndim, const int *shape, const double *origin,
+                   const double *gridsize, const int *new_shape, int method, double *out,
+                   void *workspace, size_t workspace_bytes, void *stream);
+
+/* ---------------------------------------------------------------------------------- */
 /* Interpolation                                                                       */
 /* ---------------------------------------------------------------------------------- */
 
