@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=3,
                     help="timed forward+adjoint iterations of the gradient workload (0 = skip)")
     ap.add_argument("--adj-shape", type=str, default="128,256,256")
+    ap.add_argument("--no-analytic", action="store_true",
+                    help="skip the error report against closed-form traveltimes")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only copy GIB GiB once on the device (torch clone) and exit: the "
                          "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -135,6 +137,7 @@ def main():
     del out, tt
     torch.cuda.empty_cache()
     adj = fwd_adjoint(args, torch, fteikpy_amd, synthetic, dev) if args.adj_steps > 0 else None
+    ana = None if args.no_analytic else analytic_errors(torch, fteikpy_amd, dev)
 
     nodes = nz * nx * ny
     alg_bytes = sets_total * 8 * nodes * BYTES_PER_NSU
@@ -211,6 +214,7 @@ def main():
                              "unchanged nodes are skipped"},
         "cpu_baseline": cpu,
         "fwd_adjoint": adj,
+        "analytic": ana,
     }
     print(json.dumps(res), flush=True)
 
@@ -233,6 +237,40 @@ def copy_calibration(gib):
     dt = time.perf_counter() - t0
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "seconds": dt,
                       "checksum": float(y[::4096].sum())}), flush=True)
+
+
+def analytic_errors(torch, fa, dev):
+    """Error of converged solves against closed forms on the 200^3, 5 m grid of the
+    single-source configuration (source at (502.3, 497.1, 503.9) m): a homogeneous medium
+    (3000 m/s, t = |x - x_s| / v) and a constant vertical gradient v = v0 + g z (1800 m/s
+    at the top, g = 2.7 1/s: 1800..4500 m/s over the model; cell velocities at cell
+    centres; t = arccosh(1 + g^2 R^2 / (2 v(z_s) v(z))) / g).  Errors over the nodes more
+    than 10 cells from the source; FTeik is first order, so they shrink with the spacing."""
+    n, h = 200, 5.0
+    src = np.array([502.3, 497.1, 503.9])
+    ax = torch.arange(n + 1, dtype=torch.float64, device=dev) * h
+    z = ax[:, None, None]
+    r2 = (z - src[0]) ** 2 + (ax[None, :, None] - src[1]) ** 2 + (ax[None, None, :] - src[2]) ** 2
+    far = r2.sqrt() > 10 * h
+    out = {"grid": [n, n, n], "spacing_m": h, "source_m": src.tolist(),
+           "nodes_excluded_within_m": 10 * h}
+    v0, g = 1800.0, 2.7
+    zc = (np.arange(n) + 0.5) * h
+    cases = {"homogeneous_3000": (np.full((n, n, n), 3000.0),
+                                  lambda: r2.sqrt() / 3000.0),
+             "gradient_1800_plus_2.7z": (
+                 np.ascontiguousarray(np.broadcast_to((v0 + g * zc)[:, None, None], (n, n, n))),
+                 lambda: torch.acosh(1.0 + g * g * r2 / (2.0 * (v0 + g * src[0]) * (v0 + g * z))) / g)}
+    for name, (v, exact) in cases.items():
+        tt, _, _, sets = fa.Eikonal3D(v, (h, h, h)).solve_tensors(src, nsweep=None)
+        t = exact()
+        err = (tt[0] - t).abs()[far]
+        rel = err / t[far]
+        out[name] = {"sets": int(sets[0]), "max_abs_s": float(err.max()),
+                     "max_rel": float(rel.max()), "mean_rel": float(rel.mean())}
+        del tt
+    log("analytic errors")
+    return out
 
 
 def fwd_adjoint(args, torch, fa, synthetic, dev):

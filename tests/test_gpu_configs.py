@@ -1,0 +1,109 @@
+"""GPU checks at the benchmark configurations' full sizes.
+
+- 2D, 101 x 301 gradient model with 4 Gaussian anomalies, off-grid source at (12.3 m,
+  1013.7 m): nsweep = 2 and converged, with gradients and 200 receivers, bit-identical to
+  the oracle (small enough to run whole on the CPU);
+- 3D, 200^3 layered model with a smooth perturbation, off-grid source: the first two sweep
+  sets bit-identical to the oracle at full size, the converged grid a fixed point (one
+  more set changes nothing) that the grid two sets earlier bounds from above, 10,000
+  receivers interpolated as the oracle interpolates the same grid;
+- the error against closed forms: homogeneous medium (t = s |x - x_s|) and constant
+  gradient (t = arccosh(1 + g^2 R^2 / (2 v_s v)) / g), first order in the spacing.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from fteikpy_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def fa(device):
+    import fteikpy_amd
+
+    return fteikpy_amd
+
+
+def test_config1_2d_full_size(fa):
+    v, d, src, rec = synthetic.model_2d_gaussians()
+    eik = fa.Eikonal2D(v, d)
+    axes = [np.arange(n + 1) * di for n, di in zip(v.shape, d)]
+    for nsweep in (2, None):
+        print(f"2D 101x301 nsweep={nsweep}", flush=True)
+        tt = eik.solve(src, nsweep=nsweep, return_gradient=True)
+        ref, rgrad, rvz, k = oracle.solve2d(1.0 / v, *d, src, nsweep=nsweep or 0, grad=True)
+        assert np.array_equal(tt.grid, ref)
+        if nsweep is None:
+            assert tt.sets_done == k
+        np.testing.assert_allclose(np.stack([g.grid for g in tt.gradient], -1), rgrad,
+                                   rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(tt(rec), oracle.vinterp2d(*axes, ref, rec, src, rvz),
+                                   rtol=1e-13, atol=0)
+
+
+@pytest.fixture(scope="module")
+def layered(fa):
+    v, d, src, rec = synthetic.model_3d_layers()
+    return v, d, src, rec, fa.Eikonal3D(v, d)
+
+
+def test_config2_first_sets_bitwise(fa, layered):
+    v, d, src, _, eik = layered
+    ref = oracle.solve3d(1.0 / v, *d, src, nsweep=2, order=1)[0]
+    print("3D 200^3: two sets against the oracle", flush=True)
+    assert np.array_equal(eik.solve(src, nsweep=2).grid, ref)
+
+
+def test_config2_converged_fixed_point_and_receivers(fa, layered):
+    v, d, src, rec, eik = layered
+    tt = eik.solve(src, nsweep=None)
+    k = tt.sets_done
+    print(f"3D 200^3 converged after {k} sets", flush=True)
+    assert 2 < k < 100
+    # set k is the first that changes nothing: k - 1 sets reach the fixed point, one more
+    # set leaves it, and set k - 1 still lowered some node
+    assert np.array_equal(eik.solve(src, nsweep=k - 1).grid, tt.grid)
+    assert np.array_equal(eik.solve(src, nsweep=k + 1).grid, tt.grid)
+    before = eik.solve(src, nsweep=k - 2).grid
+    assert (before >= tt.grid).all() and (before > tt.grid).any()
+    axes = [np.arange(n + 1) * di for n, di in zip(v.shape, d)]
+    np.testing.assert_allclose(tt(rec), oracle.vinterp3d(*axes, tt.grid, rec, src, tt.vzero),
+                               rtol=1e-13, atol=0)
+
+
+def _homogeneous_error(fa, n, h, vel=3000.0):
+    v = np.full((n, n, n), vel)
+    src = np.array([0.5, 0.37, 0.61]) * n * h
+    tt = fa.Eikonal3D(v, (h, h, h)).solve(src, nsweep=None).grid
+    ax = np.arange(n + 1) * h
+    r = np.sqrt((ax[:, None, None] - src[0]) ** 2 + (ax[None, :, None] - src[1]) ** 2
+                + (ax[None, None, :] - src[2]) ** 2)
+    far = r > 0.25 * n * h
+    return np.abs(tt - r / vel)[far].max()
+
+
+def test_homogeneous_first_order_convergence(fa):
+    """Max error away from the source at spacing h and h/2 on the same domain: FTeik is
+    first order, so halving h must cut it clearly (ratio > 1.5)."""
+    e1 = _homogeneous_error(fa, 24, 20.0)
+    e2 = _homogeneous_error(fa, 48, 10.0)
+    print(f"homogeneous max error: h=20 {e1:.3e}, h=10 {e2:.3e}", flush=True)
+    assert e2 < e1 / 1.5
+
+
+def test_constant_gradient_error(fa):
+    n, h, v0, g = 40, 10.0, 2000.0, 1.0
+    zc = (np.arange(n) + 0.5) * h
+    v = np.broadcast_to((v0 + g * zc)[:, None, None], (n, n, n)).copy()
+    src = np.array([3.0 * h + 1.7, 20.3 * h, 19.6 * h])
+    tt = fa.Eikonal3D(v, (h, h, h)).solve(src, nsweep=None).grid
+    ax = np.arange(n + 1) * h
+    z = ax[:, None, None]
+    r2 = (z - src[0]) ** 2 + (ax[None, :, None] - src[1]) ** 2 + (ax[None, None, :] - src[2]) ** 2
+    t = np.arccosh(1.0 + g * g * r2 / (2.0 * (v0 + g * src[0]) * (v0 + g * z))) / g
+    far = np.sqrt(r2) > 10 * h
+    rel = np.abs(tt - t)[far] / t[far]
+    print(f"constant gradient: max rel error {rel.max():.3e}, mean {rel.mean():.3e}", flush=True)
+    assert rel.max() < 0.05 and rel.mean() < 0.02
