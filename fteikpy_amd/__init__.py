@@ -9,6 +9,7 @@ from ._adjoint import receiver_traveltimes
 from ._batch import solve_batch, solve_batch_tensors
 from ._grid import Grid2D, Grid3D, TraveltimeGrid2D, TraveltimeGrid3D
 from ._helpers import get_num_threads, set_num_threads
+from ._io import grid_to_meshio, ray_to_meshio
 from ._lib import build as _build_lib
 from ._solver import Eikonal2D, Eikonal3D
 
@@ -18,16 +19,6 @@ __version__ = "0.1.0"
 def build():
     """Compile the HIP library (libfteik_amd.so, gfx950) in-tree."""
     _build_lib()
-
-
-def grid_to_meshio(*args):
-    """meshio export (fteikpy/_io.py:7-106) is not part of this build."""
-    raise NotImplementedError("meshio export is not implemented in fteikpy_amd")
-
-
-def ray_to_meshio(*args):
-    """meshio export (fteikpy/_io.py:109-153) is not part of this build."""
-    raise NotImplementedError("meshio export is not implemented in fteikpy_amd")
 
 
 __all__ = [

@@ -8,7 +8,8 @@ slowness.  The best pair depends on the workload:
   bricks skip more of the sparse later sets (a brick is skipped only when none of its
   nodes or upwind neighbours changed in the last 8 passes);
 - a workgroup's waves meet at every step, so wide groups wait for their slowest source;
-- a single source is latency bound and prefers short bricks.
+- a single source is latency bound and prefers short bricks;
+- the library's own policy (brick height 0) changes the height between sweep sets.
 
 ``tune3d`` times converged solves of the caller's own workload for a list of candidates
 (first the brick heights at a fixed group, then the groups at the best height) and records
@@ -24,7 +25,7 @@ import numpy as np
 
 _CHOICES = {}
 
-TILES = (8, 16, 32, 48, 64, 0)  # 0 = the whole z column
+TILES = (8, 16, 32, 48, 64, -1, 0)  # -1: the whole z column; 0: the library's policy
 GROUPS = (1, 2, 4)
 
 
@@ -58,8 +59,10 @@ def load(path):
 
 
 def default3d(nsrc):
-    """The library's defaults for nsrc sources (fteik3d_solve with tile_a = group = 0)."""
-    return (32 if nsrc >= 32 else 8), (2 if nsrc >= 8 else 1)
+    """The library's defaults for nsrc sources (fteik3d_solve with tile_a = group = 0):
+    brick height 0 = its per-phase policy (from 32 sources on: whole columns in the first
+    set, 32 planes after it; a fixed 8 below 32 sources)."""
+    return 0, (2 if nsrc >= 8 else 1)
 
 
 def tune3d(eik, sources, *, tiles=TILES, groups=GROUPS, max_sets=100, use_graph=True,
@@ -77,7 +80,7 @@ def tune3d(eik, sources, *, tiles=TILES, groups=GROUPS, max_sets=100, use_graph=
     src = np.asarray(sources, dtype=np.float64).reshape(-1, 3)
     n = len(src)
     nz, nx, ny = eik.shape
-    tiles = sorted({min(int(t), nz + 1) if t > 0 else nz + 1 for t in tiles})
+    tiles = sorted({min(int(t), nz + 1) if t > 0 else (nz + 1 if t < 0 else 0) for t in tiles})
     groups = [g for g in groups if g <= max(1, n)] or [1]
     out = torch.empty((n, nz + 1, nx + 1, ny + 1), dtype=torch.float64,
                       device=torch.device("cuda", torch.cuda.current_device()))
@@ -99,7 +102,6 @@ def tune3d(eik, sources, *, tiles=TILES, groups=GROUPS, max_sets=100, use_graph=
         return best
 
     dt, dg = default3d(n)
-    dt = min(dt, nz + 1)
     g0 = dg if dg in groups else groups[0]
     run(tiles[0], g0)  # warm-up: first launches, graph machinery
     timings.clear()

@@ -566,6 +566,28 @@ __global__ void zero_bytes_kernel(uint8_t *p, long long n) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
 }
 
+// Tile bytes of another brick height from the per-node change bytes (the OR of a tile's
+// node bytes), for the sources still changing: one wave per (tile, listed source).
+__global__ void tsum_rebuild_kernel(const uint8_t *chg, uint8_t *tsum, const int *list,
+                                    const int *nlist, long long node_stride, int NZ, int NX,
+                                    int NY, int tile_a, int nbb, int nbc, int ntiles) {
+    const int tile = blockIdx.x, li = blockIdx.y;
+    if (li >= *nlist) return;
+    const int src = list[li];
+    const int tk = tile % nbc, tj = (tile / nbc) % nbb, ti = tile / (nbc * nbb);
+    const int i0 = ti * tile_a, i1 = min(NZ, i0 + tile_a);
+    const int lane = threadIdx.x;
+    const int j = tj * 8 + (lane >> 3), k = tk * 8 + (lane & 7);
+    unsigned acc = 0;
+    if (j < NX && k < NY) {
+        const uint8_t *c = chg + (long long)src * node_stride + (long long)j * NY + k;
+        for (int i = i0; i < i1; ++i) acc |= c[(long long)i * NX * NY];
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc |= __shfl_xor(acc, o);
+    if (lane == 0) tsum[(long long)src * ntiles + tile] = (uint8_t)acc;
+}
+
 __device__ __forceinline__ double t_ana3(int i, int j, int k, double dz, double dx, double dy,
                                          const SrcParam &s, double vzero) {
     const double a = dz * ((double)i - s.zsa), b = dx * ((double)j - s.xsa),
@@ -927,19 +949,37 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
 
     const int NZ = nz + 1, NX = nx + 1, NY = ny + 1;
     const long long nn = (long long)NZ * NX * NY;
-    Ws3 w;
-    SweepArgs a;
-    make_args(slow, nz, nx, ny, dz, dx, dy, nsrc, tile_a, group, want_grad ? sgn : nullptr, tt,
-              sets_done, (char *)workspace, &a, &w);
-    a.log_node = hv ? hv->node : nullptr;
-    a.log_old = hv ? hv->old : nullptr;
-    a.log_count = hv ? &hv->hdr->count : nullptr;
-    a.log_cap = hv ? hcap : 0;
-    std::vector<int> nb(a.ndiag), cum(a.ndiag + 1, 0);
-    for (int d = 0; d < a.ndiag; ++d) {
-        nb[d] = bricks_on_diag(d, a.nba, a.nbb, a.nbc);
-        cum[d + 1] = cum[d] + nb[d];
+    // Brick height per set.  A height given by the caller is used throughout.  Otherwise,
+    // for batches (>= 32 sources): whole columns in the first set (every node is evaluated;
+    // the lane skew costs least), then 32 planes (shorter bricks are skipped more often in
+    // the sparse later sets) -- measured on MI355X (an 8-plane phase for the last few
+    // sources measured slower).  The tile bytes of the second height are rebuilt from the
+    // per-node change bytes at the switch.
+    int geo_tile[2];
+    const bool adaptive = tile_a <= 0 && nsrc >= 32 && schedule == 1;
+    if (adaptive) {
+        geo_tile[0] = NZ;
+        geo_tile[1] = default_tile(nsrc);
+    } else {
+        geo_tile[0] = geo_tile[1] = tile_a > 0 ? tile_a : default_tile(nsrc);
     }
+    Ws3 w;
+    SweepArgs geo[2];
+    std::vector<int> geo_nb[2];
+    for (int g = 0; g < 2; ++g) {
+        make_args(slow, nz, nx, ny, dz, dx, dy, nsrc, geo_tile[g], group,
+                  want_grad ? sgn : nullptr, tt, sets_done, (char *)workspace, &geo[g], &w);
+        geo[g].log_node = hv ? hv->node : nullptr;
+        geo[g].log_old = hv ? hv->old : nullptr;
+        geo[g].log_count = hv ? &hv->hdr->count : nullptr;
+        geo[g].log_cap = hv ? hcap : 0;
+        geo_nb[g].resize(geo[g].ndiag);
+        for (int d = 0; d < geo[g].ndiag; ++d)
+            geo_nb[g][d] = bricks_on_diag(d, geo[g].nba, geo[g].nbb, geo[g].nbc);
+    }
+    SweepArgs &a = geo[0];
+    std::vector<int> cum(a.ndiag + 1, 0);
+    for (int d = 0; d < a.ndiag; ++d) cum[d + 1] = cum[d] + geo_nb[0][d];
     const long long gmax = a.gmax;
     if (8LL * a.ntiles * gmax >= (1LL << 31)) return FTEIK_ERR_ARG;  // task index is an int
     const int pblocks = schedule == 0 ? pers_blocks(want_grad != 0, group) : 0;
@@ -982,7 +1022,9 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
         }
     }
 
-    auto enqueue_set = [&]() -> int {
+    auto enqueue_set = [&](int g) -> int {
+        SweepArgs &a = geo[g];
+        const std::vector<int> &nb = geo_nb[g];
         if (schedule == 0) {
             const long long np = (long long)a.ntiles * gmax;
             hipLaunchKernelGGL(set_reset_kernel,
@@ -1012,36 +1054,59 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
         return hip_status(hipGetLastError());
     };
 
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
-    if (use_graph && stream) {  // the legacy default stream cannot be captured
-        FTEIK_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        int st = enqueue_set();
-        hipError_t e = hipStreamEndCapture(stream, &graph);
-        if (st) return st;
-        FTEIK_CHECK(e);
-        FTEIK_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    }
-    auto run_set = [&]() -> int {
-        if (exec) return hip_status(hipGraphLaunch(exec, stream));
-        return enqueue_set();
+    // one captured sweep set per brick height, instantiated on first use
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    auto run_set = [&](int g) -> int {
+        if (use_graph && stream) {  // the legacy default stream cannot be captured
+            if (!exec[g]) {
+                FTEIK_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                int st = enqueue_set(g);
+                hipError_t e = hipStreamEndCapture(stream, &graph[g]);
+                if (st) return st;
+                FTEIK_CHECK(e);
+                FTEIK_CHECK(hipGraphInstantiate(&exec[g], graph[g], nullptr, nullptr, 0));
+            }
+            return hip_status(hipGraphLaunch(exec[g], stream));
+        }
+        return enqueue_set(g);
+    };
+    int cur = 0;  // brick height of the previous set
+    auto switch_to = [&](int g) -> int {
+        if (geo[g].tile_a == geo[cur].tile_a) {
+            cur = g;
+            return 0;
+        }
+        const SweepArgs &b = geo[g];
+        hipLaunchKernelGGL(tsum_rebuild_kernel, dim3((unsigned)b.ntiles, (unsigned)nsrc), dim3(64),
+                           0, stream, (const uint8_t *)w.chg, w.tsum, (const int *)w.list,
+                           (const int *)w.nlist, nn, NZ, NX, NY, b.tile_a, b.nbb, b.nbc,
+                           b.ntiles);
+        cur = g;
+        return hip_status(hipGetLastError());
     };
 
     int status = 0;
     if (nsweep > 0) {
-        for (int s = 0; s < nsweep && !status; ++s) status = run_set();
+        for (int s = 0; s < nsweep && !status; ++s) {
+            status = switch_to(s == 0 ? 0 : 1);
+            if (!status) status = run_set(cur);
+        }
     } else {
         int nact = nsrc;
         for (int s = 0; s < max_sets && nact > 0 && !status; ++s) {
-            status = run_set();
+            status = switch_to(s == 0 ? 0 : 1);
+            if (!status) status = run_set(cur);
             if (status) break;
             hipError_t e = hipMemcpyAsync(&nact, w.nlist, sizeof(int), hipMemcpyDeviceToHost, stream);
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
             status = hip_status(e);
         }
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    for (int g = 0; g < 2; ++g) {
+        if (exec[g]) (void)hipGraphExecDestroy(exec[g]);
+        if (graph[g]) (void)hipGraphDestroy(graph[g]);
+    }
     if (!status && schedule == 0) {
         // a dependency wait that gave up leaves a wrong result: report it
         int err = 0;

@@ -227,3 +227,23 @@ def test_many_sets_skip_bitwise(fa):
         g = fa.TraveltimeGrid3D(tt[1], d, np.zeros(3), srcs[1], None, vz[1], sign=sg[1])
         np.testing.assert_allclose(np.stack([c.grid for c in g.gradient], -1), r[1][1],
                                    rtol=1e-13, atol=1e-15)
+
+
+def test_brick_height_policy_bitwise(fa):
+    """From 32 sources on, the default brick height changes after the first set (whole
+    columns, then 32 planes) with the tile bytes rebuilt at the switch: every intermediate
+    nsweep and the converged grids stay bit-identical to the oracle."""
+    shape, d = (45, 27, 30), (9.0, 8.0, 10.0)  # 46 node planes: whole column != 32
+    v = _rough_model(shape, 17)
+    rng = np.random.default_rng(17)
+    srcs = rng.random((40, 3)) * np.array(shape) * d
+    eik = fa.Eikonal3D(v, d)
+    ref, _, _, kref = oracle.solve3d(1.0 / v, *d, srcs, nsweep=0)
+    assert kref.max() > kref.min()  # sources retire at different sets
+    for use_graph in (True, False):
+        tt, _, _, sets = eik.solve_tensors(srcs, nsweep=None, use_graph=use_graph)
+        assert list(sets) == list(kref)
+        assert np.array_equal(tt.cpu().numpy(), ref)
+    for ns in (1, 2, 3):
+        r = oracle.solve3d(1.0 / v, *d, srcs, nsweep=ns)[0]
+        assert np.array_equal(eik.solve_tensors(srcs, nsweep=ns, use_graph=True)[0].cpu().numpy(), r)
