@@ -67,6 +67,9 @@ struct SweepArgs {
     uint8_t *sgn;
     uint8_t *chg;           // per node: bit p set if the node changed in the last pass p
     uint8_t *tsum;          // per source and tile: OR of the tile nodes' change bits
+    uint64_t *faces;        // per source and tile: byte f = OR of the change bits of the
+                            // tile's face f (0/1 first/last node plane, 2/3 first/last x
+                            // column, 4/5 first/last y column); null: tile bytes only
     int *changed;
     const int *list;        // sources still changing (compacted), first *nlist entries
     const int *nlist;
@@ -164,14 +167,25 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
         const int s0 = has ? A.list[g0 + wave] : 0;
         bool on = has;
         if (has && A.sets_done[s0] > 0) {
-            // lanes 0..7 read the tile byte and its 7 upwind neighbours
+            // lane 0 reads the tile byte, lanes 1..7 what its 7 upwind neighbour tiles hold
+            // next to it: the face toward this tile (its z face for the neighbours below or
+            // above, its x face for the x and xy neighbours, a superset of the edge or
+            // corner they share), or their whole-tile byte without face bytes
             unsigned any = 0;
             if (lane < 8) {
                 const int ii = ti - ((lane & 1) ? sz : 0);
                 const int jj = tj - ((lane & 2) ? sx : 0);
                 const int kk = tk - ((lane & 4) ? sy : 0);
-                if (ii >= 0 && ii < A.nba && jj >= 0 && jj < A.nbb && kk >= 0 && kk < A.nbc)
-                    any = A.tsum[s0 * tiles + ((long long)ii * A.nbb + jj) * A.nbc + kk];
+                if (ii >= 0 && ii < A.nba && jj >= 0 && jj < A.nbb && kk >= 0 && kk < A.nbc) {
+                    const long long t = s0 * tiles + ((long long)ii * A.nbb + jj) * A.nbc + kk;
+                    if (lane == 0 || A.faces == nullptr) {
+                        any = A.tsum[t];
+                    } else {
+                        const int f = (lane & 1) ? (sz > 0 ? 1 : 0)
+                                                 : ((lane & 2) ? (sx > 0 ? 3 : 2) : (sy > 0 ? 5 : 4));
+                        any = (unsigned)(A.faces[t] >> (8 * f)) & 0xffu;
+                    }
+                }
             }
             on = __any(any != 0u);
         }
@@ -304,6 +318,10 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
     unsigned hist = 0;   // bit k: this lane changed its node k steps ago
     unsigned chist = 0;  // bit k: this lane changed its node's change bits k steps ago
     bool any_change = false;
+    // face bytes of this tile: final change bytes of its nodes on the first/last node
+    // plane, and (per lane, applied at the end) of the lane's column
+    unsigned face_zlo = 0, face_zhi = 0, col_or = 0;
+    const int tile_ilo = ti * A.tile_a, tile_ihi = min(A.NZ, tile_ilo + A.tile_a) - 1;
     const int nsteps = na + kSkew;
     // ring slots advance by one per step: this lane's current plane p = s - bb - cc sits in
     // slot (p+1) mod kRing, the plane written back at step s in slot (s-kSkew+1) mod kRing
@@ -384,9 +402,14 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
             // write back the plane every lane has finished (only what changed)
             const int pw = s - kSkew;
             if (pw >= 0 && pw < na && inb) {
-                const long long g = (long long)node_i(a_first + pw) * plane_nodes + col_off;
+                const int mi = node_i(a_first + pw);
+                const long long g = (long long)mi * plane_nodes + col_off;
                 if ((hist >> (kSkew - bb - cc)) & 1u) tt[g] = ring[wslot][tblk + self];
-                if ((chist >> (kSkew - bb - cc)) & 1u) cg[g] = ringc[wslot][tblk + self];
+                const unsigned cfin = ringc[wslot][tblk + self];
+                if ((chist >> (kSkew - bb - cc)) & 1u) cg[g] = (uint8_t)cfin;
+                col_or |= cfin;
+                if (mi == tile_ilo) face_zlo |= cfin;
+                if (mi == tile_ihi) face_zhi |= cfin;
             }
             cslot = cslot == kRing - 1 ? 0 : cslot + 1;
             wslot = wslot == kRing - 1 ? 0 : wslot + 1;
@@ -401,6 +424,25 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
     }
 
     if (LOG) log_pad(A, log_pos, log_end, lane);
+    if (A.faces != nullptr && has_src) {
+        // every node on a face another tile reads was processed in this pass (only nodes
+        // on the grid's boundary faces are not), so the bytes are complete
+        const int mj = inb ? node_j(b) : -1, mk = inb ? node_k(c) : -1;
+        unsigned f[6] = {face_zlo, face_zhi,
+                         mj == tj * kLanesB ? col_or : 0u,
+                         mj == min(NX, tj * kLanesB + kLanesB) - 1 ? col_or : 0u,
+                         mk == tk * kLanesC ? col_or : 0u,
+                         mk == min(NY, tk * kLanesC + kLanesC) - 1 ? col_or : 0u};
+        unsigned long long packed = 0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            unsigned v = f[q];
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) v |= __shfl_xor(v, o);
+            packed |= (unsigned long long)(v & 0xffu) << (8 * q);
+        }
+        if (lane == 0) A.faces[src * tiles + ((long long)ti * A.nbb + tj) * A.nbc + tk] = packed;
+    }
     if (has_src) {
         const bool wave_changed = __any(any_change);
         if (lane == 0) {
@@ -568,9 +610,10 @@ __global__ void zero_bytes_kernel(uint8_t *p, long long n) {
 
 // Tile bytes of another brick height from the per-node change bytes (the OR of a tile's
 // node bytes), for the sources still changing: one wave per (tile, listed source).
-__global__ void tsum_rebuild_kernel(const uint8_t *chg, uint8_t *tsum, const int *list,
-                                    const int *nlist, long long node_stride, int NZ, int NX,
-                                    int NY, int tile_a, int nbb, int nbc, int ntiles) {
+__global__ void tsum_rebuild_kernel(const uint8_t *chg, uint8_t *tsum, uint64_t *faces,
+                                    const int *list, const int *nlist, long long node_stride,
+                                    int NZ, int NX, int NY, int tile_a, int nbb, int nbc,
+                                    int ntiles) {
     const int tile = blockIdx.x, li = blockIdx.y;
     if (li >= *nlist) return;
     const int src = list[li];
@@ -578,14 +621,28 @@ __global__ void tsum_rebuild_kernel(const uint8_t *chg, uint8_t *tsum, const int
     const int i0 = ti * tile_a, i1 = min(NZ, i0 + tile_a);
     const int lane = threadIdx.x;
     const int j = tj * 8 + (lane >> 3), k = tk * 8 + (lane & 7);
-    unsigned acc = 0;
+    unsigned acc = 0, zlo = 0, zhi = 0;
     if (j < NX && k < NY) {
         const uint8_t *c = chg + (long long)src * node_stride + (long long)j * NY + k;
         for (int i = i0; i < i1; ++i) acc |= c[(long long)i * NX * NY];
+        zlo = c[(long long)i0 * NX * NY];
+        zhi = c[(long long)(i1 - 1) * NX * NY];
     }
+    unsigned f[7] = {acc, zlo, zhi, j == tj * 8 ? acc : 0u, j == min(NX, tj * 8 + 8) - 1 ? acc : 0u,
+                     k == tk * 8 ? acc : 0u, k == min(NY, tk * 8 + 8) - 1 ? acc : 0u};
+    if (!(j < NX && k < NY)) f[3] = f[4] = f[5] = f[6] = 0u;
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) acc |= __shfl_xor(acc, o);
-    if (lane == 0) tsum[(long long)src * ntiles + tile] = (uint8_t)acc;
+    for (int q = 0; q < 7; ++q)
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) f[q] |= __shfl_xor(f[q], o);
+    if (lane == 0) {
+        tsum[(long long)src * ntiles + tile] = (uint8_t)f[0];
+        if (faces) {
+            unsigned long long packed = 0;
+            for (int q = 0; q < 6; ++q) packed |= (unsigned long long)(f[q + 1] & 0xffu) << (8 * q);
+            faces[(long long)src * ntiles + tile] = packed;
+        }
+    }
 }
 
 __device__ __forceinline__ double t_ana3(int i, int j, int k, double dz, double dx, double dy,
@@ -790,6 +847,7 @@ struct Ws3 {
     SrcParam *prm;
     int *changed, *active, *list, *nlist, *queue, *error, *cum_nb, *progress;
     uint8_t *chg, *tsum;
+    uint64_t *faces;
 };
 
 // Upper bounds over every tile_a >= 1: tiles of the memory grid and anti-diagonals.
@@ -799,6 +857,14 @@ size_t max_tiles(int nz, int nx, int ny) {
 }
 size_t max_diag(int nz, int nx, int ny) {
     return (size_t)(nz + 1) + (nx + 1 + kLanesB - 1) / kLanesB + (ny + 1 + kLanesC - 1) / kLanesC;
+}
+
+// Face bytes are kept for bricks of at least kFaceMinTile planes (tiles of any smaller
+// height use the whole-tile bytes of their neighbours).
+constexpr int kFaceMinTile = 8;
+size_t face_tiles(int nz, int nx, int ny) {
+    return (size_t)((nz + 1 + kFaceMinTile - 1) / kFaceMinTile) *
+           ((nx + 1 + kLanesB - 1) / kLanesB) * ((ny + 1 + kLanesC - 1) / kLanesC);
 }
 
 size_t ws_bytes(int nz, int nx, int ny, int nsrc, Ws3 *w, char *base) {
@@ -819,6 +885,7 @@ size_t ws_bytes(int nz, int nx, int ny, int nsrc, Ws3 *w, char *base) {
     char *prog = take(sizeof(int) * tiles * nsrc);
     char *chg = take(nodes * nsrc);
     char *tsum = take(tiles * nsrc);
+    char *faces = take(sizeof(uint64_t) * face_tiles(nz, nx, ny) * nsrc);
     if (w) {
         w->prm = (SrcParam *)prm;
         w->changed = (int *)changed;
@@ -831,6 +898,7 @@ size_t ws_bytes(int nz, int nx, int ny, int nsrc, Ws3 *w, char *base) {
         w->progress = (int *)prog;
         w->chg = (uint8_t *)chg;
         w->tsum = (uint8_t *)tsum;
+        w->faces = (uint64_t *)faces;
     }
     return off;
 }
@@ -855,6 +923,7 @@ void make_args(const double *slow, int nz, int nx, int ny, double dz, double dx,
     a.sgn = sgn;
     a.chg = w.chg;
     a.tsum = w.tsum;
+    a.faces = nullptr;
     a.changed = w.changed;
     a.list = w.list;
     a.nlist = w.nlist;
@@ -881,6 +950,7 @@ void make_args(const double *slow, int nz, int nx, int ny, double dz, double dx,
     a.log_cap = 0;
     a.wait_ticks = 0;
     a.gmax = ceil_div(nsrc, group);
+    if (a.tile_a >= kFaceMinTile) a.faces = w.faces;
     a.dz = dz; a.dx = dx; a.dy = dy;
     a.dz2i = 1.0 / dz / dz;
     a.dx2i = 1.0 / dx / dx;
@@ -1007,6 +1077,13 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
         hipLaunchKernelGGL(zero_bytes_kernel, dim3((unsigned)std::min<long long>((nt + 255) / 256, 65536)),
                            dim3(256), 0, stream, w.tsum, nt);
         FTEIK_CHECK(hipGetLastError());
+        if (a.faces) {
+            const long long nf = nt * (long long)sizeof(uint64_t);
+            hipLaunchKernelGGL(zero_bytes_kernel,
+                               dim3((unsigned)std::min<long long>((nf + 255) / 256, 65536)),
+                               dim3(256), 0, stream, (uint8_t *)a.faces, nf);
+            FTEIK_CHECK(hipGetLastError());
+        }
         hipLaunchKernelGGL(seed3d_kernel, dim3((nsrc * 8 + 255) / 256), dim3(256), 0, stream,
                            slow, nz, nx, ny, dz, dx, dy, w.prm, nsrc, tt, nn, vzero, sets_done,
                            w.changed, w.active, w.list, w.nlist);
@@ -1079,7 +1156,7 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
         }
         const SweepArgs &b = geo[g];
         hipLaunchKernelGGL(tsum_rebuild_kernel, dim3((unsigned)b.ntiles, (unsigned)nsrc), dim3(64),
-                           0, stream, (const uint8_t *)w.chg, w.tsum, (const int *)w.list,
+                           0, stream, (const uint8_t *)w.chg, w.tsum, b.faces, (const int *)w.list,
                            (const int *)w.nlist, nn, NZ, NX, NY, b.tile_a, b.nbb, b.nbc,
                            b.ntiles);
         cur = g;
@@ -1361,6 +1438,13 @@ int batch3_run(Batch3 *b, int use_graph, hipStream_t stream) {
                            dim3((unsigned)std::min<long long>((nt + 255) / 256, 65536)), dim3(256),
                            0, stream, a.tsum, nt);
         FTEIK_CHECK(hipGetLastError());
+        if (a.faces) {
+            const long long nf = nt * (long long)sizeof(uint64_t);
+            hipLaunchKernelGGL(zero_bytes_kernel,
+                               dim3((unsigned)std::min<long long>((nf + 255) / 256, 65536)),
+                               dim3(256), 0, stream, (uint8_t *)a.faces, nf);
+            FTEIK_CHECK(hipGetLastError());
+        }
         hipLaunchKernelGGL(seed3d_kernel, dim3((q.nsrc * 8 + 255) / 256), dim3(256), 0, stream,
                            q.slow, q.nz, q.nx, q.ny, q.dz, q.dx, q.dy, b->ws[m].prm, q.nsrc, q.tt,
                            a.node_stride, q.vzero, q.sets_done, a.changed, b->ws[m].active,
