@@ -33,7 +33,7 @@ BYTES_PER_NSU = 24  # FP64 tt read + tt write + slowness read per node update
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Commands and figures:
 # profiles/sweep3d_traffic__inclusions_160x320x320_256src.md
 PMC_TRAFFIC = {"grid": (160, 320, 320), "sources": 256, "tile_a": 0, "group": 2,
-               "fetch_bytes": 201207.79 * 2.0 * 1024, "write_bytes": 8377.95 * 1.0 * 1024}
+               "fetch_bytes": 169255.91 * 2.0 * 1024, "write_bytes": 8454.37 * 1.0 * 1024}
 
 
 def main():

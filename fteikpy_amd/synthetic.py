@@ -148,3 +148,18 @@ def uneven_batch(seed=1004, n2=16, n3=32, side2=(64, 1024), side3=((48, 192), (4
         src = rng.uniform(0.0, 1.0, size=(ns, len(shape))) * ext
         out.append((v, (d,) * len(shape), src))
     return out
+
+
+def marmousi_like(nz=300, nx=1000, d=0.01):
+    """A layered, faulted 2D model of the Marmousi model's size and units (300 x 1000 cells
+    of 10 m given in km and km/s, 1.5 to 4.7 km/s): 12 interfaces dipping and undulating
+    across the model, offset by two normal faults, velocity increasing downwards."""
+    z = (np.arange(nz) + 0.5)[:, None] * d
+    x = (np.arange(nx) + 0.5)[None, :] * d
+    v = np.full((nz, nx), 1.5)
+    vel = np.linspace(1.7, 4.7, 12)
+    for k in range(12):
+        depth = 0.15 + 0.23 * k + 0.04 * x + 0.06 * np.sin(2.0 * np.pi * x / (3.0 + 0.4 * k))
+        depth = depth + 0.12 * (x > 3.7) + 0.09 * (x > 6.4) * (k % 2)
+        v = np.where(z >= depth, vel[k], v)
+    return v, (d, d)

@@ -107,3 +107,34 @@ def test_constant_gradient_error(fa):
     rel = np.abs(tt - t)[far] / t[far]
     print(f"constant gradient: max rel error {rel.max():.3e}, mean {rel.mean():.3e}", flush=True)
     assert rel.max() < 0.05 and rel.mean() < 0.02
+
+
+def test_marmousi_like_workflow(fa):
+    """The reference's demo pipeline (.github/sample.py) on a layered, faulted model of the
+    Marmousi model's size: smooth(0.05) -> solve((0, 0), nsweep=3, gradients) -> 100 rays
+    from the surface -> traveltimes along the rays, each step against the oracle."""
+    from oracle import gridops
+
+    v, d = synthetic.marmousi_like()
+    eik = fa.Eikonal2D(v, d)
+    eik.smooth(0.05)
+    vs = gridops.smooth(v, d, 0.05)
+    assert np.array_equal(eik.grid, vs)
+    print("marmousi-like: solve", flush=True)
+    tt = eik.solve((0.0, 0.0), nsweep=3, return_gradient=True)
+    ref, rgrad, rvz, _ = oracle.solve2d(1.0 / vs, *d, np.zeros(2), nsweep=3, grad=True)
+    assert np.array_equal(tt.grid, ref)
+    ends = np.zeros((100, 2))
+    ends[:, 1] = np.linspace(4.4, eik.xaxis[-1], 100)
+    rays = tt.raytrace(ends)
+    z, x = [np.arange(n + 1) * di for n, di in zip(v.shape, d)]
+    gz, gx = rgrad[..., 0], rgrad[..., 1]
+    # the reference's default max_step (fteikpy/_grid.py:242-251): node counts x spacing
+    nzn, nxn = len(z), len(x)
+    max_step = int(2.0 * ((nzn * d[0]) ** 2 + (nxn * d[1]) ** 2) ** 0.5 / min(d))
+    ref_rays = oracle.ray2d(z, x, gz, gx, ends, np.zeros(2), min(d), max_step)
+    assert [len(r) for r in rays] == [len(r) for r in ref_rays]
+    for r, rr in zip(rays, ref_rays):
+        np.testing.assert_allclose(r, rr, rtol=1e-10, atol=1e-10)
+        np.testing.assert_allclose(tt(r), oracle.vinterp2d(z, x, ref, rr, np.zeros(2), rvz),
+                                   rtol=1e-10, atol=1e-12)
