@@ -791,6 +791,18 @@ int bricks_on_diag(int d, int nba, int nbb, int nbc) {
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
+// Captured sweep sets (one per brick height), destroyed on every return path.
+struct Graphs {
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    ~Graphs() {
+        for (int g = 0; g < 2; ++g) {
+            if (exec[g]) (void)hipGraphExecDestroy(exec[g]);
+            if (graph[g]) (void)hipGraphDestroy(graph[g]);
+        }
+    }
+};
+
 template <bool G, int W>
 void launch_diag_t(dim3 grid, hipStream_t st, const SweepArgs &a) {
     hipLaunchKernelGGL((sweep3d_diag_kernel<G, false, W>), grid, dim3(64 * W), 0, st, a);
@@ -1132,19 +1144,18 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
     };
 
     // one captured sweep set per brick height, instantiated on first use
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
-    hipGraph_t graph[2] = {nullptr, nullptr};
+    Graphs gr;
     auto run_set = [&](int g) -> int {
         if (use_graph && stream) {  // the legacy default stream cannot be captured
-            if (!exec[g]) {
+            if (!gr.exec[g]) {
                 FTEIK_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
                 int st = enqueue_set(g);
-                hipError_t e = hipStreamEndCapture(stream, &graph[g]);
+                hipError_t e = hipStreamEndCapture(stream, &gr.graph[g]);
                 if (st) return st;
                 FTEIK_CHECK(e);
-                FTEIK_CHECK(hipGraphInstantiate(&exec[g], graph[g], nullptr, nullptr, 0));
+                FTEIK_CHECK(hipGraphInstantiate(&gr.exec[g], gr.graph[g], nullptr, nullptr, 0));
             }
-            return hip_status(hipGraphLaunch(exec[g], stream));
+            return hip_status(hipGraphLaunch(gr.exec[g], stream));
         }
         return enqueue_set(g);
     };
@@ -1179,10 +1190,6 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
             status = hip_status(e);
         }
-    }
-    for (int g = 0; g < 2; ++g) {
-        if (exec[g]) (void)hipGraphExecDestroy(exec[g]);
-        if (graph[g]) (void)hipGraphDestroy(graph[g]);
     }
     if (!status && schedule == 0) {
         // a dependency wait that gave up leaves a wrong result: report it
@@ -1466,18 +1473,17 @@ int batch3_run(Batch3 *b, int use_graph, hipStream_t stream) {
         return hip_status(hipGetLastError());
     };
 
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
+    Graphs gr;
     if (use_graph && stream) {
         FTEIK_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         int st = enqueue_set();
-        hipError_t e = hipStreamEndCapture(stream, &graph);
+        hipError_t e = hipStreamEndCapture(stream, &gr.graph[0]);
         if (st) return st;
         FTEIK_CHECK(e);
-        FTEIK_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        FTEIK_CHECK(hipGraphInstantiate(&gr.exec[0], gr.graph[0], nullptr, nullptr, 0));
     }
     auto run_set = [&]() -> int {
-        if (exec) return hip_status(hipGraphLaunch(exec, stream));
+        if (gr.exec[0]) return hip_status(hipGraphLaunch(gr.exec[0], stream));
         return enqueue_set();
     };
     int status = 0;
@@ -1497,8 +1503,6 @@ int batch3_run(Batch3 *b, int use_graph, hipStream_t stream) {
             if (!any) break;
         }
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
     return status;
 }
 
