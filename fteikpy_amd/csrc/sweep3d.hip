@@ -21,7 +21,11 @@
 // node whose 7 upwind neighbours all read zero is skipped (its candidates equal those of
 // its previous update in this direction and its own time can only have decreased).  Per
 // tile, one byte ORs its nodes' bytes; a tile whose byte and its upwind neighbours' bytes
-// are zero is skipped whole.
+// are zero is skipped whole.  In the first set the bytes only accumulate (the seeded
+// corners start with all bits set), so a zero byte means a node still at its initial 1e5:
+// a node whose 7 upwind neighbours all hold 1e5 builds only candidates >= 1e5 and keeps its
+// time, and the same skips apply from the first set on (see guard3d_kernel for the bound
+// on the slowness this needs).
 //
 // Schedules.  "persistent": one launch per sweep set; resident workgroups take (pass,
 // brick, source group) tasks from a queue in a topological order and wait on per-tile
@@ -74,6 +78,8 @@ struct SweepArgs {
     const int *list;        // sources still changing (compacted), first *nlist entries
     const int *nlist;
     const int *sets_done;   // sets completed per source (0 during the first set)
+    const int *guard;       // 0: the first set may skip (every cell's slowness is large
+                            // enough for untouched neighbours to stay untouched)
     int *progress;          // persistent: passes finished per (source group, tile)
     int *queue;             // persistent: next task
     int *error;             // persistent: set when a dependency wait gave up
@@ -166,7 +172,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
         const bool has = g0 + wave < nact;
         const int s0 = has ? A.list[g0 + wave] : 0;
         bool on = has;
-        if (has && A.sets_done[s0] > 0) {
+        if (has && (A.sets_done[s0] > 0 || *A.guard == 0)) {
             // lane 0 reads the tile byte, lanes 1..7 what its 7 upwind neighbour tiles hold
             // next to it: the face toward this tile (its z face for the neighbours below or
             // above, its x face for the x and xy neighbours, a superset of the edge or
@@ -201,7 +207,10 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
     const int src = has_src ? A.list[g0 + wave] : 0;
     const int bb = lane >> 3, cc = lane & 7;
     const int b = b0 + bb, c = c0 + cc;
-    const bool inb = has_src && b >= 1 && b <= NX - 1 && c >= 1 && c <= NY - 1;
+    // ing: this lane's column is in the grid; inb: its nodes are updated in this pass (the
+    // oriented planes b = 0 and c = 0 are not)
+    const bool ing = has_src && b >= 0 && b <= NX - 1 && c >= 0 && c <= NY - 1;
+    const bool inb = ing && b >= 1 && c >= 1;
 
     // oriented -> memory index maps
     auto node_i = [&](int a) { return sz > 0 ? a : A.NZ - 1 - a; };
@@ -300,8 +309,12 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
     // From the second set on, a node none of whose 7 upwind neighbours changed during the
     // last 8 passes sees exactly the candidates of its previous update in this direction,
     // while its own time can only have decreased since: it cannot change and is skipped.
-    const bool may_skip = has_src && A.sets_done[src] > 0;
+    // In the first set the bytes only accumulate, and the same test finds 7 neighbours at
+    // their initial time (valid when the guard holds).
+    const bool first = !has_src || A.sets_done[src] == 0;
+    const bool may_skip = has_src && (!first || *A.guard == 0);
     const unsigned pbit = 1u << pass;
+    const unsigned keep = first ? 0xffu : ~pbit;  // bits kept besides this pass's
     uint8_t *cg = A.chg + (long long)src * A.node_stride;
 
     const int tblk = wave * kPlane, sblk = kT;
@@ -322,6 +335,15 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
     // plane, and (per lane, applied at the end) of the lane's column
     unsigned face_zlo = 0, face_zhi = 0, col_or = 0;
     const int tile_ilo = ti * A.tile_a, tile_ihi = min(A.NZ, tile_ilo + A.tile_a) - 1;
+    if (a0 <= 0 && ing) {
+        // the tile holds the oriented plane a = 0, which no pass in this direction updates:
+        // its bytes (upwind plane, slot 0) still belong in the faces other tiles read
+        const unsigned c0b = ringc[0][tblk + self];
+        const int mi = node_i(0);
+        col_or |= c0b;
+        if (mi == tile_ilo) face_zlo |= c0b;
+        if (mi == tile_ihi) face_zhi |= c0b;
+    }
     const int nsteps = na + kSkew;
     // ring slots advance by one per step: this lane's current plane p = s - bb - cc sits in
     // slot (p+1) mod kRing, the plane written back at step s in slot (s-kSkew+1) mod kRing
@@ -364,7 +386,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
                     }
                 }
                 const unsigned cs = curc[self];
-                const unsigned cn = (cs & ~pbit) | (changed_now ? pbit : 0u);
+                const unsigned cn = (cs & keep) | (changed_now ? pbit : 0u);
                 if (cn != cs) {
                     curc[self] = (uint8_t)cn;
                     cchanged = true;
@@ -401,12 +423,14 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
 
             // write back the plane every lane has finished (only what changed)
             const int pw = s - kSkew;
-            if (pw >= 0 && pw < na && inb) {
+            if (pw >= 0 && pw < na && ing) {
                 const int mi = node_i(a_first + pw);
-                const long long g = (long long)mi * plane_nodes + col_off;
-                if ((hist >> (kSkew - bb - cc)) & 1u) tt[g] = ring[wslot][tblk + self];
                 const unsigned cfin = ringc[wslot][tblk + self];
-                if ((chist >> (kSkew - bb - cc)) & 1u) cg[g] = (uint8_t)cfin;
+                if (inb) {
+                    const long long g = (long long)mi * plane_nodes + col_off;
+                    if ((hist >> (kSkew - bb - cc)) & 1u) tt[g] = ring[wslot][tblk + self];
+                    if ((chist >> (kSkew - bb - cc)) & 1u) cg[g] = (uint8_t)cfin;
+                }
                 col_or |= cfin;
                 if (mi == tile_ilo) face_zlo |= cfin;
                 if (mi == tile_ihi) face_zhi |= cfin;
@@ -425,9 +449,8 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
 
     if (LOG) log_pad(A, log_pos, log_end, lane);
     if (A.faces != nullptr && has_src) {
-        // every node on a face another tile reads was processed in this pass (only nodes
-        // on the grid's boundary faces are not), so the bytes are complete
-        const int mj = inb ? node_j(b) : -1, mk = inb ? node_k(c) : -1;
+        // the bytes of every node of the tile went in, updated in this pass or not
+        const int mj = ing ? node_j(b) : -1, mk = ing ? node_k(c) : -1;
         unsigned f[6] = {face_zlo, face_zhi,
                          mj == tj * kLanesB ? col_or : 0u,
                          mj == min(NX, tj * kLanesB + kLanesB) - 1 ? col_or : 0u,
@@ -449,7 +472,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
             if (wave_changed) A.changed[src] = 1;
             uint8_t *ts = A.tsum + src * tiles + ((long long)ti * A.nbb + tj) * A.nbc + tk;
             const unsigned old = *ts;
-            const unsigned nw = (old & ~pbit) | (wave_changed ? pbit : 0u);
+            const unsigned nw = (old & keep) | (wave_changed ? pbit : 0u);
             if (nw != old) *ts = (uint8_t)nw;
         }
     }
@@ -653,31 +676,70 @@ __device__ __forceinline__ double t_ana3(int i, int j, int k, double dz, double 
 }
 
 // Seeds the 8 corners of each source cell (_fteik3d.py:439-456), records vzero, resets
-// the per-source set bookkeeping.  One thread per (source, corner).
-__global__ void seed3d_kernel(const double *slow, int nz, int nx, int ny, double dz, double dx,
-                              double dy, const SrcParam *prm, int nsrc, double *tt,
-                              long long node_stride, double *vzero, int *sets_done,
-                              int *changed, int *active, int *list, int *nlist) {
+// the per-source set bookkeeping.  The corners' change bytes, and the tile and face bytes
+// of the tiles of the first set's geometry that hold them, start with every bit set; the
+// first-set guard starts cleared.  One thread per (source, corner).
+struct SeedArgs {
+    const SrcParam *prm;
+    double *tt;
+    double *vzero;
+    int *sets_done, *changed, *active, *list, *nlist;
+    uint8_t *chg, *tsum;
+    uint64_t *faces;      // null: no face bytes in the first set
+    int *guard;
+    long long node_stride;
+    int nsrc, nz, nx, ny;
+    int tile_a, nbb, nbc, ntiles;  // first set's tile geometry
+    double dz, dx, dy;
+};
+
+__global__ void seed3d_kernel(const double *slow, SeedArgs A) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nsrc * 8) return;
+    if (g >= A.nsrc * 8) return;
     const int s = g >> 3, c = g & 7;
-    const SrcParam P = prm[s];
+    const SrcParam P = A.prm[s];
+    const int nx = A.nx, ny = A.ny;
     const double vz = slow[((long long)P.zsi * nx + P.xsi) * ny + P.ysi];
     // corner order of the reference: (0,0,0) (1,0,0) (0,1,0) (0,0,1) (1,1,0) (1,0,1) (0,1,1) (1,1,1)
     const int di = (c == 1 || c == 4 || c == 5 || c == 7);
     const int dj = (c == 2 || c == 4 || c == 6 || c == 7);
     const int dk = (c == 3 || c == 5 || c == 6 || c == 7);
     const int i = P.zsi + di, j = P.xsi + dj, k = P.ysi + dk;
-    tt[(long long)s * node_stride + ((long long)i * (nx + 1) + j) * (ny + 1) + k] =
-        t_ana3(i, j, k, dz, dx, dy, P, vz);
+    const long long n = (long long)s * A.node_stride + ((long long)i * (nx + 1) + j) * (ny + 1) + k;
+    A.tt[n] = t_ana3(i, j, k, A.dz, A.dx, A.dy, P, vz);
+    A.chg[n] = 0xff;
+    const long long t = (long long)s * A.ntiles +
+                        ((long long)(i / A.tile_a) * A.nbb + j / kLanesB) * A.nbc + k / kLanesC;
+    A.tsum[t] = 0xff;
+    if (A.faces) A.faces[t] = ~0ull;
     if (c == 0) {
-        vzero[s] = vz;
-        sets_done[s] = 0;
-        changed[s] = 0;
-        active[s] = 1;
-        list[s] = s;
-        if (s == 0) *nlist = nsrc;
+        A.vzero[s] = vz;
+        A.sets_done[s] = 0;
+        A.changed[s] = 0;
+        A.active[s] = 1;
+        A.list[s] = s;
+        if (s == 0) {
+            *A.nlist = A.nsrc;
+            *A.guard = 0;
+        }
     }
+}
+
+// First-set skip guard.  A node whose 7 upwind neighbours all hold the initial 1e5 builds
+// candidates 1e5 + dz vref (1D), 1e5 + 2 vref dz dx / sqrt(dz^2 + dx^2) (2D, likewise for
+// the other planes) and 1e5 + 3 vref / sqrt(1/dz^2 + 1/dx^2 + 1/dy^2) (3D), each added term
+// >= sqrt(2) vref min(dz, dx, dy) in exact arithmetic, while rounding the 2D and 3D forms
+// moves them by at most ~10 ulp(1e5) = 1.5e-10.  With every cell's slowness times the
+// smallest spacing at least 1e-9 (vref is a cell slowness), every candidate stays >= 1e5 and
+// the node keeps its time, so it may be skipped.  A cell below that (or not a number) sets
+// the guard and the first set updates every node.
+constexpr double kGuardTime = 1e-9;
+__global__ void guard3d_kernel(const double *slow, long long ncell, double hmin, int *guard) {
+    bool bad = false;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
+         c += (long long)gridDim.x * blockDim.x)
+        bad |= !(slow[c] * hmin >= kGuardTime);
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(guard, 1);
 }
 
 // After each set: count it for every active source, retire sources whose set changed
@@ -857,7 +919,7 @@ int pers_blocks(bool grad, int group) {
 
 struct Ws3 {
     SrcParam *prm;
-    int *changed, *active, *list, *nlist, *queue, *error, *cum_nb, *progress;
+    int *changed, *active, *list, *nlist, *queue, *error, *guard, *cum_nb, *progress;
     uint8_t *chg, *tsum;
     uint64_t *faces;
 };
@@ -892,7 +954,7 @@ size_t ws_bytes(int nz, int nx, int ny, int nsrc, Ws3 *w, char *base) {
     char *changed = take(sizeof(int) * nsrc);
     char *active = take(sizeof(int) * nsrc);
     char *list = take(sizeof(int) * nsrc);
-    char *misc = take(sizeof(int) * 8);  // nlist, queue, error
+    char *misc = take(sizeof(int) * 8);  // nlist, queue, error, guard
     char *cum = take(sizeof(int) * (max_diag(nz, nx, ny) + 1));
     char *prog = take(sizeof(int) * tiles * nsrc);
     char *chg = take(nodes * nsrc);
@@ -906,6 +968,7 @@ size_t ws_bytes(int nz, int nx, int ny, int nsrc, Ws3 *w, char *base) {
         w->nlist = (int *)misc;
         w->queue = (int *)misc + 1;
         w->error = (int *)misc + 2;
+        w->guard = (int *)misc + 3;
         w->cum_nb = (int *)cum;
         w->progress = (int *)prog;
         w->chg = (uint8_t *)chg;
@@ -913,6 +976,37 @@ size_t ws_bytes(int nz, int nx, int ny, int nsrc, Ws3 *w, char *base) {
         w->faces = (uint64_t *)faces;
     }
     return off;
+}
+
+// Seeds every source of one model and evaluates the first-set guard (after the byte
+// arrays were cleared).  `a` carries the first set's tile geometry.
+int launch_seed3d(const SweepArgs &a, const Ws3 &w, const double *slow, int nsrc, double *vzero,
+                  int *sets_done, hipStream_t stream) {
+    SeedArgs sa;
+    sa.prm = w.prm;
+    sa.tt = a.tt;
+    sa.vzero = vzero;
+    sa.sets_done = sets_done;
+    sa.changed = w.changed;
+    sa.active = w.active;
+    sa.list = w.list;
+    sa.nlist = w.nlist;
+    sa.chg = w.chg;
+    sa.tsum = w.tsum;
+    sa.faces = a.faces;
+    sa.guard = w.guard;
+    sa.node_stride = a.node_stride;
+    sa.nsrc = nsrc;
+    sa.nz = a.nz; sa.nx = a.nx; sa.ny = a.ny;
+    sa.tile_a = a.tile_a; sa.nbb = a.nbb; sa.nbc = a.nbc; sa.ntiles = a.ntiles;
+    sa.dz = a.dz; sa.dx = a.dx; sa.dy = a.dy;
+    hipLaunchKernelGGL(seed3d_kernel, dim3((nsrc * 8 + 255) / 256), dim3(256), 0, stream, slow, sa);
+    FTEIK_CHECK(hipGetLastError());
+    const long long ncell = (long long)a.nz * a.nx * a.ny;
+    const double hmin = std::min(a.dz, std::min(a.dx, a.dy));
+    hipLaunchKernelGGL(guard3d_kernel, dim3((unsigned)std::min<long long>((ncell + 255) / 256, 2048)),
+                       dim3(256), 0, stream, slow, ncell, hmin, w.guard);
+    return hip_status(hipGetLastError());
 }
 
 // brick height default: 32-plane bricks for batches (shorter bricks skip more of the sparse
@@ -940,6 +1034,7 @@ void make_args(const double *slow, int nz, int nx, int ny, double dz, double dx,
     a.list = w.list;
     a.nlist = w.nlist;
     a.sets_done = sets_done;
+    a.guard = w.guard;
     a.progress = w.progress;
     a.queue = w.queue;
     a.error = w.error;
@@ -1096,10 +1191,7 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
                                dim3(256), 0, stream, (uint8_t *)a.faces, nf);
             FTEIK_CHECK(hipGetLastError());
         }
-        hipLaunchKernelGGL(seed3d_kernel, dim3((nsrc * 8 + 255) / 256), dim3(256), 0, stream,
-                           slow, nz, nx, ny, dz, dx, dy, w.prm, nsrc, tt, nn, vzero, sets_done,
-                           w.changed, w.active, w.list, w.nlist);
-        FTEIK_CHECK(hipGetLastError());
+        if (int st = launch_seed3d(a, w, slow, nsrc, vzero, sets_done, stream)) return st;
         hipLaunchKernelGGL(zero_bytes_kernel, dim3(1), dim3(64), 0, stream, (uint8_t *)w.error,
                            (long long)sizeof(int));
         FTEIK_CHECK(hipGetLastError());
@@ -1452,11 +1544,8 @@ int batch3_run(Batch3 *b, int use_graph, hipStream_t stream) {
                                dim3(256), 0, stream, (uint8_t *)a.faces, nf);
             FTEIK_CHECK(hipGetLastError());
         }
-        hipLaunchKernelGGL(seed3d_kernel, dim3((q.nsrc * 8 + 255) / 256), dim3(256), 0, stream,
-                           q.slow, q.nz, q.nx, q.ny, q.dz, q.dx, q.dy, b->ws[m].prm, q.nsrc, q.tt,
-                           a.node_stride, q.vzero, q.sets_done, a.changed, b->ws[m].active,
-                           b->ws[m].list, b->ws[m].nlist);
-        FTEIK_CHECK(hipGetLastError());
+        if (int st = launch_seed3d(a, b->ws[m], q.slow, q.nsrc, q.vzero, q.sets_done, stream))
+            return st;
     }
 
     const int dmax = b->L.dmax;

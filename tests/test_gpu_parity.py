@@ -247,3 +247,39 @@ def test_brick_height_policy_bitwise(fa):
     for ns in (1, 2, 3):
         r = oracle.solve3d(1.0 / v, *d, srcs, nsweep=ns)[0]
         assert np.array_equal(eik.solve_tensors(srcs, nsweep=ns, use_graph=True)[0].cpu().numpy(), r)
+
+
+@pytest.mark.parametrize("tile_a, group", [(8, 2), (16, 1), (0, 0)])
+def test_first_set_skip_boundary_sources_bitwise(fa, tile_a, group):
+    """The first set already skips nodes whose 7 upwind neighbours still hold 1e5, and tile
+    faces carry the bytes of nodes a pass does not update (the grid's first plane in that
+    direction): sources on the surface, on edges and in corners, where those planes are
+    reached first, stay bit-identical after 1 and 2 sets and at convergence."""
+    shape, d = (26, 21, 19), (8.0, 10.0, 9.0)
+    v = _rough_model(shape, 23)
+    v[:2] *= 0.4  # slow surface layer: head waves run along the first node planes
+    ext = np.array(shape) * d
+    srcs = np.array([[0.0, 101.3, 77.7], [0.0, 0.0, 0.0], [ext[0], ext[1], 0.0],
+                     [113.0, 0.0, ext[2]], [0.0, ext[1], 45.5], [96.1, 117.0, 83.2]])
+    eik = fa.Eikonal3D(v, d)
+    for ns in (1, 2):
+        r = oracle.solve3d(1.0 / v, *d, srcs, nsweep=ns)[0]
+        tt = eik.solve_tensors(srcs, nsweep=ns, tile_a=tile_a, group=group)[0]
+        assert np.array_equal(tt.cpu().numpy(), r), ns
+    ref, _, _, kref = oracle.solve3d(1.0 / v, *d, srcs, nsweep=0)
+    tt, _, _, sets = eik.solve_tensors(srcs, nsweep=None, tile_a=tile_a, group=group)
+    assert list(sets) == list(kref)
+    assert np.array_equal(tt.cpu().numpy(), ref)
+
+
+def test_first_set_guard_tiny_slowness_bitwise(fa):
+    """Slowness x spacing below 1e-9 s turns the first-set skipping off (candidates built
+    from untouched neighbours could round below 1e5 there): results stay bit-identical."""
+    shape, d = (17, 15, 20), (1.0, 1.0, 1.0)
+    v = 1.0e12 * _rough_model(shape, 29) / 1500.0
+    srcs = np.array([[3.2, 7.7, 11.1], [0.0, 15.0, 20.0]])
+    eik = fa.Eikonal3D(v, d)
+    for ns in (1, 0):
+        ref, _, _, kref = oracle.solve3d(1.0 / v, *d, srcs, nsweep=ns)
+        tt, _, _, sets = eik.solve_tensors(srcs, nsweep=ns or None, tile_a=8, group=2)
+        assert np.array_equal(tt.cpu().numpy(), ref), ns
