@@ -66,7 +66,7 @@ __device__ __forceinline__ int dir_sz(int p) { return (p & 4) ? -1 : 1; }
 __device__ __forceinline__ int dir_sx(int p) { return (p & 1) ? -1 : 1; }
 __device__ __forceinline__ int dir_sy(int p) { return (p & 2) ? -1 : 1; }
 
-struct AdjArgs {
+struct AdjArgs : Stencil3 {
     const double *slow;
     const double *tt;  // [nsrc][NZ][NX][NY]
     const AdjSrc *src;
@@ -79,7 +79,6 @@ struct AdjArgs {
     int *stale;        // stale nodes carrying adjoint
     long long nodes;
     int nz, nx, ny, NZ, NX, NY, nsrc;
-    double dz, dx, dy, dz2i, dx2i, dy2i, dz2dx2, dz2dy2, dx2dy2, dsum;
 };
 
 __device__ __forceinline__ long long node_at(const AdjArgs &A, int i, int j, int k) {
@@ -541,10 +540,7 @@ void fill_args(AdjArgs &A, const double *slow, int nz, int nx, int ny, double dz
     A.cells = (long long)nz * nx * ny;
     A.ngroup = dlds_groups(nsrc);
     A.nz = nz; A.nx = nx; A.ny = ny; A.NZ = nz + 1; A.NX = nx + 1; A.NY = ny + 1; A.nsrc = nsrc;
-    A.dz = dz; A.dx = dx; A.dy = dy;
-    A.dz2i = 1.0 / dz / dz; A.dx2i = 1.0 / dx / dx; A.dy2i = 1.0 / dy / dy;
-    A.dz2dx2 = A.dz2i * A.dx2i; A.dz2dy2 = A.dz2i * A.dy2i; A.dx2dy2 = A.dx2i * A.dy2i;
-    A.dsum = A.dz2i + A.dx2i + A.dy2i;
+    static_cast<Stencil3 &>(A) = make_stencil3(dz, dx, dy);
 }
 
 // Placement of every source from its absolute position (relative = absolute - origin,

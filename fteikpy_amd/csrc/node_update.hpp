@@ -1,10 +1,69 @@
 // FTeik 3D node operator shared by the sweep kernels and the adjoint's stencil selection.
-// Args supplies dz, dx, dy, dz2i, dx2i, dy2i, dz2dx2, dz2dy2, dx2dy2 and dsum.
 #pragma once
 
 #include "fteik_device.hpp"
 
 namespace fteik {
+
+// Grid constants of the 3D operator, fixed for a solve (computed on the host).
+struct Stencil3 {
+    double dz, dx, dy, dz2i, dx2i, dy2i, dz2dx2, dz2dy2, dx2dy2, dsum;
+    double szx, szy, sxy;        // dz2i + dx2i, dz2i + dy2i, dx2i + dy2i: 2D denominators
+    double szx4, szy4, sxy4;     // 4x those (exact scaling)
+    double rzx, rzy, rxy, rsum;  // 1 / szx, 1 / szy, 1 / sxy, 1 / dsum, correctly rounded
+};
+
+inline Stencil3 make_stencil3(double dz, double dx, double dy) {
+    Stencil3 c;
+    c.dz = dz; c.dx = dx; c.dy = dy;
+    c.dz2i = 1.0 / dz / dz;
+    c.dx2i = 1.0 / dx / dx;
+    c.dy2i = 1.0 / dy / dy;
+    c.dz2dx2 = c.dz2i * c.dx2i;
+    c.dz2dy2 = c.dz2i * c.dy2i;
+    c.dx2dy2 = c.dx2i * c.dy2i;
+    c.dsum = c.dz2i + c.dx2i + c.dy2i;
+    c.szx = c.dz2i + c.dx2i;
+    c.szy = c.dz2i + c.dy2i;
+    c.sxy = c.dx2i + c.dy2i;
+    c.szx4 = 4.0 * c.szx;
+    c.szy4 = 4.0 * c.szy;
+    c.sxy4 = 4.0 * c.sxy;
+    c.rzx = 1.0 / c.szx;
+    c.rzy = 1.0 / c.szy;
+    c.rxy = 1.0 / c.sxy;
+    c.rsum = 1.0 / c.dsum;
+    return c;
+}
+
+// a / b, correctly rounded, for a divisor b fixed per solve with y = 1 / b correctly
+// rounded.  q0 = a y is within 2 ulp of a / b; one correction with the exact remainder
+// a - b q0 (an FMA) brings it within 1 ulp, and by Markstein's theorem (y within half an
+// ulp of 1 / b, q within 1 ulp of a / b) a second one, q + (a - b q) y rounded once, is the
+// correctly rounded quotient.  Five dependent operations instead of the generic division's
+// scaled reciprocal refinement.  Valid away from overflow and the subnormal range, which
+// traveltimes and inverse squared spacings stay far from.
+__device__ __forceinline__ double div_fixed(double a, double b, double y) {
+    const double q0 = a * y;
+    const double q1 = fma(fma(-b, q0, a), y, q0);
+    return fma(fma(-b, q1, a), y, q1);
+}
+
+// A copy of the constants held in vector registers for the whole sweep loop: as uniform
+// values they would take ~46 scalar registers (the loop's bookkeeping already fills the
+// scalar file) and, with one scalar operand allowed per vector instruction, extra moves.
+// The empty asm only pins each value to a VGPR pair.
+__device__ __forceinline__ void pin_vgpr(double &x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ Stencil3 stencil_in_vgprs(const Stencil3 &a) {
+    Stencil3 c = a;
+    pin_vgpr(c.dz); pin_vgpr(c.dx); pin_vgpr(c.dy);
+    pin_vgpr(c.dz2i); pin_vgpr(c.dx2i); pin_vgpr(c.dy2i);
+    pin_vgpr(c.dz2dx2); pin_vgpr(c.dz2dy2); pin_vgpr(c.dx2dy2); pin_vgpr(c.dsum);
+    pin_vgpr(c.szx); pin_vgpr(c.szy); pin_vgpr(c.sxy);
+    pin_vgpr(c.szx4); pin_vgpr(c.szy4); pin_vgpr(c.sxy4);
+    pin_vgpr(c.rzx); pin_vgpr(c.rzy); pin_vgpr(c.rxy); pin_vgpr(c.rsum);
+    return c;
+}
 
 __device__ __forceinline__ double min4(double a, double b, double c, double d) {
     return fmin(fmin(a, b), fmin(c, d));
@@ -15,6 +74,7 @@ __device__ __forceinline__ double min4(double a, double b, double c, double d) {
 // diagonal ones; sXYZ: slowness of the cell at (a-1 | a') x (b-1 | b') x (c-1 | c'),
 // with ' the downwind-side cell clamped to the grid (the reference's max(i-1,0) /
 // min(i, n-2) pair).  Returns the new value; *which receives the winning candidate.
+// Args derives from Stencil3.
 template <class Args>
 __device__ __forceinline__ double node_update(const Args &A, double t0, double tv, double te,
                                               double tn, double tev, double ten, double tnv,
@@ -23,6 +83,9 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
                                               int *which) {
     const double dz = A.dz, dx = A.dx, dy = A.dy;
     const double dz2i = A.dz2i, dx2i = A.dx2i, dy2i = A.dy2i;
+    // the reference's (dz2i + dx2i), dz2i * dx2i, 4.0 * (vref * vref) * (dz2i + dx2i) and
+    // x / (dz2i + dx2i) are evaluated from the precomputed sums, products and reciprocals:
+    // the same doubles (4x scalings are exact)
 
     // 1D operators (refracted times), _fteik3d.py:78-104
     const double t1d1 = tv + dz * min4(s000, s001, s010, s011);
@@ -36,27 +99,27 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
         const double vref = fmin(s000, s001);  // ZX plane
         if (tv < te + dx * vref && te < tv + dz * vref) {
             const double ta = tev + te - tv, tb = tev - te + tv, q = ta - tb;
-            t2d1 = ((tb * dz2i + ta * dx2i) +
-                    sqrt(4.0 * (vref * vref) * (dz2i + dx2i) - dz2i * dx2i * (q * q))) /
-                   (dz2i + dx2i);
+            t2d1 = div_fixed((tb * dz2i + ta * dx2i) +
+                                 sqrt((vref * vref) * A.szx4 - A.dz2dx2 * (q * q)),
+                             A.szx, A.rzx);
         }
     }
     {
         const double vref = fmin(s000, s010);  // ZY plane
         if (tv < tn + dy * vref && tn < tv + dz * vref) {
             const double ta = tv - tn + tnv, tb = tn - tv + tnv, q = ta - tb;
-            t2d2 = ((ta * dz2i + tb * dy2i) +
-                    sqrt(4.0 * (vref * vref) * (dz2i + dy2i) - dz2i * dy2i * (q * q))) /
-                   (dz2i + dy2i);
+            t2d2 = div_fixed((ta * dz2i + tb * dy2i) +
+                                 sqrt((vref * vref) * A.szy4 - A.dz2dy2 * (q * q)),
+                             A.szy, A.rzy);
         }
     }
     {
         const double vref = fmin(s000, s100);  // XY plane
         if (te < tn + dy * vref && tn < te + dx * vref) {
             const double ta = te - tn + ten, tb = tn - te + ten, q = ta - tb;
-            t2d3 = ((ta * dx2i + tb * dy2i) +
-                    sqrt(4.0 * (vref * vref) * (dx2i + dy2i) - dx2i * dy2i * (q * q))) /
-                   (dx2i + dy2i);
+            t2d3 = div_fixed((ta * dx2i + tb * dy2i) +
+                                 sqrt((vref * vref) * A.sxy4 - A.dx2dy2 * (q * q)),
+                             A.sxy, A.rxy);
         }
     }
     const double t2d = fmin(fmin(t2d1, t2d2), t2d3);
@@ -73,7 +136,7 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
         double t3 = A.dz2dx2 * (q1 * q1);
         t3 += A.dz2dy2 * (q2 * q2);
         t3 += A.dx2dy2 * (q3 * q3);
-        if (t2 >= t3) t3d = (tb * dz2i + ta * dx2i + tc * dy2i + sqrt(t2 - t3)) / A.dsum;
+        if (t2 >= t3) t3d = div_fixed(tb * dz2i + ta * dx2i + tc * dy2i + sqrt(t2 - t3), A.dsum, A.rsum);
     }
 
     const double tnew = fmin(fmin(fmin(t0, t1d), t2d), t3d);

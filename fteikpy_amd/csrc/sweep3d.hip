@@ -65,7 +65,7 @@ __host__ __device__ constexpr int bpitch(int n) {
     return 4 * ((n + 3) / 4 + ((8 - ((n + 3) / 4) % 16) + 16) % 16);
 }
 
-struct SweepArgs {
+struct SweepArgs : Stencil3 {
     const double *slow;
     double *tt;
     uint8_t *sgn;
@@ -98,7 +98,6 @@ struct SweepArgs {
     double *log_old;        // ... and its time before the change
     unsigned long long *log_count;
     long long log_cap;
-    double dz, dx, dy, dz2i, dx2i, dy2i, dz2dx2, dz2dy2, dx2dy2, dsum;
 };
 
 // pass order of the reference (_fteik3d.py:217-391): z flips at pass 4, y at 2, x at 1
@@ -317,6 +316,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
     const unsigned keep = first ? 0xffu : ~pbit;  // bits kept besides this pass's
     uint8_t *cg = A.chg + (long long)src * A.node_stride;
 
+    const Stencil3 C = stencil_in_vgprs(A);
     const int tblk = wave * kPlane, sblk = kT;
     const int self = (bb + 1) * 9 + (cc + 1);  // this lane's node in a block
     const int up_b = bb * 9 + (cc + 1);        // b-1
@@ -368,7 +368,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
                 if (!may_skip || up != 0u) {
                     int which;
                     const double tnew = node_update(
-                        A, t0, prv[tblk + self], cur[tblk + up_b], cur[tblk + up_c],
+                        C, t0, prv[tblk + self], cur[tblk + up_b], cur[tblk + up_c],
                         prv[tblk + up_b], cur[tblk + up_bc], prv[tblk + up_c], prv[tblk + up_bc],
                         prv[sblk + up_bc], prv[sblk + up_b], prv[sblk + up_c], prv[sblk + self],
                         cur[sblk + up_bc], cur[sblk + up_b], cur[sblk + up_c], &which);
@@ -1058,14 +1058,7 @@ void make_args(const double *slow, int nz, int nx, int ny, double dz, double dx,
     a.wait_ticks = 0;
     a.gmax = ceil_div(nsrc, group);
     if (a.tile_a >= kFaceMinTile) a.faces = w.faces;
-    a.dz = dz; a.dx = dx; a.dy = dy;
-    a.dz2i = 1.0 / dz / dz;
-    a.dx2i = 1.0 / dx / dx;
-    a.dy2i = 1.0 / dy / dy;
-    a.dz2dx2 = a.dz2i * a.dx2i;
-    a.dz2dy2 = a.dz2i * a.dy2i;
-    a.dx2dy2 = a.dx2i * a.dy2i;
-    a.dsum = a.dz2i + a.dx2i + a.dy2i;
+    static_cast<Stencil3 &>(a) = make_stencil3(dz, dx, dy);
     *out = a;
     *wout = w;
 }
