@@ -65,6 +65,30 @@ __device__ __forceinline__ Stencil3 stencil_in_vgprs(const Stencil3 &a) {
     return c;
 }
 
+// sqrt(x), correctly rounded, with the compiler's own refinement sequence for doubles
+// without its range handling: for x in [2^-767, +inf) the library first scales only
+// below 2^-767 and special-cases only 0 and +inf, so there this is the same sequence on
+// the same value (rsq estimate, then the Goldschmidt/Newton steps) and returns the same
+// double.  Any other argument (0, tiny, infinite, negative, NaN) takes the library sqrt; the
+// branch is uniform over the wave, and the node update produces such arguments only as
+// exact zeros.
+__device__ __forceinline__ double sqrt_rn(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    const bool other = !(x >= 0x1p-767 && x < __builtin_inf());
+    if (__builtin_expect(__any(other), 0)) {
+        if (other) g = sqrt(x);
+    }
+    return g;
+}
+
 __device__ __forceinline__ double min4(double a, double b, double c, double d) {
     return fmin(fmin(a, b), fmin(c, d));
 }
@@ -100,7 +124,7 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
         if (tv < te + dx * vref && te < tv + dz * vref) {
             const double ta = tev + te - tv, tb = tev - te + tv, q = ta - tb;
             t2d1 = div_fixed((tb * dz2i + ta * dx2i) +
-                                 sqrt((vref * vref) * A.szx4 - A.dz2dx2 * (q * q)),
+                                 sqrt_rn((vref * vref) * A.szx4 - A.dz2dx2 * (q * q)),
                              A.szx, A.rzx);
         }
     }
@@ -109,7 +133,7 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
         if (tv < tn + dy * vref && tn < tv + dz * vref) {
             const double ta = tv - tn + tnv, tb = tn - tv + tnv, q = ta - tb;
             t2d2 = div_fixed((ta * dz2i + tb * dy2i) +
-                                 sqrt((vref * vref) * A.szy4 - A.dz2dy2 * (q * q)),
+                                 sqrt_rn((vref * vref) * A.szy4 - A.dz2dy2 * (q * q)),
                              A.szy, A.rzy);
         }
     }
@@ -118,7 +142,7 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
         if (te < tn + dy * vref && tn < te + dx * vref) {
             const double ta = te - tn + ten, tb = tn - te + ten, q = ta - tb;
             t2d3 = div_fixed((ta * dx2i + tb * dy2i) +
-                                 sqrt((vref * vref) * A.sxy4 - A.dx2dy2 * (q * q)),
+                                 sqrt_rn((vref * vref) * A.sxy4 - A.dx2dy2 * (q * q)),
                              A.sxy, A.rxy);
         }
     }
@@ -136,7 +160,7 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
         double t3 = A.dz2dx2 * (q1 * q1);
         t3 += A.dz2dy2 * (q2 * q2);
         t3 += A.dx2dy2 * (q3 * q3);
-        if (t2 >= t3) t3d = div_fixed(tb * dz2i + ta * dx2i + tc * dy2i + sqrt(t2 - t3), A.dsum, A.rsum);
+        if (t2 >= t3) t3d = div_fixed(tb * dz2i + ta * dx2i + tc * dy2i + sqrt_rn(t2 - t3), A.dsum, A.rsum);
     }
 
     const double tnew = fmin(fmin(fmin(t0, t1d), t2d), t3d);
