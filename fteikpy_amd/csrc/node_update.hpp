@@ -9,7 +9,6 @@ namespace fteik {
 struct Stencil3 {
     double dz, dx, dy, dz2i, dx2i, dy2i, dz2dx2, dz2dy2, dx2dy2, dsum;
     double szx, szy, sxy;        // dz2i + dx2i, dz2i + dy2i, dx2i + dy2i: 2D denominators
-    double szx4, szy4, sxy4;     // 4x those (exact scaling)
     double rzx, rzy, rxy, rsum;  // 1 / szx, 1 / szy, 1 / sxy, 1 / dsum, correctly rounded
 };
 
@@ -26,9 +25,6 @@ inline Stencil3 make_stencil3(double dz, double dx, double dy) {
     c.szx = c.dz2i + c.dx2i;
     c.szy = c.dz2i + c.dy2i;
     c.sxy = c.dx2i + c.dy2i;
-    c.szx4 = 4.0 * c.szx;
-    c.szy4 = 4.0 * c.szy;
-    c.sxy4 = 4.0 * c.sxy;
     c.rzx = 1.0 / c.szx;
     c.rzy = 1.0 / c.szy;
     c.rxy = 1.0 / c.sxy;
@@ -50,7 +46,7 @@ __device__ __forceinline__ double div_fixed(double a, double b, double y) {
 }
 
 // A copy of the constants held in vector registers for the whole sweep loop: as uniform
-// values they would take ~46 scalar registers (the loop's bookkeeping already fills the
+// values they would take ~34 scalar registers (the loop's bookkeeping already fills the
 // scalar file) and, with one scalar operand allowed per vector instruction, extra moves.
 // The empty asm only pins each value to a VGPR pair.
 __device__ __forceinline__ void pin_vgpr(double &x) { asm volatile("" : "+v"(x)); }
@@ -60,7 +56,6 @@ __device__ __forceinline__ Stencil3 stencil_in_vgprs(const Stencil3 &a) {
     pin_vgpr(c.dz2i); pin_vgpr(c.dx2i); pin_vgpr(c.dy2i);
     pin_vgpr(c.dz2dx2); pin_vgpr(c.dz2dy2); pin_vgpr(c.dx2dy2); pin_vgpr(c.dsum);
     pin_vgpr(c.szx); pin_vgpr(c.szy); pin_vgpr(c.sxy);
-    pin_vgpr(c.szx4); pin_vgpr(c.szy4); pin_vgpr(c.sxy4);
     pin_vgpr(c.rzx); pin_vgpr(c.rzy); pin_vgpr(c.rxy); pin_vgpr(c.rsum);
     return c;
 }
@@ -107,9 +102,8 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
                                               int *which) {
     const double dz = A.dz, dx = A.dx, dy = A.dy;
     const double dz2i = A.dz2i, dx2i = A.dx2i, dy2i = A.dy2i;
-    // the reference's (dz2i + dx2i), dz2i * dx2i, 4.0 * (vref * vref) * (dz2i + dx2i) and
-    // x / (dz2i + dx2i) are evaluated from the precomputed sums, products and reciprocals:
-    // the same doubles (4x scalings are exact)
+    // the reference's (dz2i + dx2i), dz2i * dx2i and x / (dz2i + dx2i) are evaluated from
+    // the precomputed sums, products and reciprocals: the same doubles
 
     // 1D operators (refracted times), _fteik3d.py:78-104
     const double t1d1 = tv + dz * min4(s000, s001, s010, s011);
@@ -124,7 +118,7 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
         if (tv < te + dx * vref && te < tv + dz * vref) {
             const double ta = tev + te - tv, tb = tev - te + tv, q = ta - tb;
             t2d1 = div_fixed((tb * dz2i + ta * dx2i) +
-                                 sqrt_rn((vref * vref) * A.szx4 - A.dz2dx2 * (q * q)),
+                                 sqrt_rn(4.0 * (vref * vref) * A.szx - A.dz2dx2 * (q * q)),
                              A.szx, A.rzx);
         }
     }
@@ -133,7 +127,7 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
         if (tv < tn + dy * vref && tn < tv + dz * vref) {
             const double ta = tv - tn + tnv, tb = tn - tv + tnv, q = ta - tb;
             t2d2 = div_fixed((ta * dz2i + tb * dy2i) +
-                                 sqrt_rn((vref * vref) * A.szy4 - A.dz2dy2 * (q * q)),
+                                 sqrt_rn(4.0 * (vref * vref) * A.szy - A.dz2dy2 * (q * q)),
                              A.szy, A.rzy);
         }
     }
@@ -142,7 +136,7 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
         if (te < tn + dy * vref && tn < te + dx * vref) {
             const double ta = te - tn + ten, tb = tn - te + ten, q = ta - tb;
             t2d3 = div_fixed((ta * dx2i + tb * dy2i) +
-                                 sqrt_rn((vref * vref) * A.sxy4 - A.dx2dy2 * (q * q)),
+                                 sqrt_rn(4.0 * (vref * vref) * A.sxy - A.dx2dy2 * (q * q)),
                              A.sxy, A.rxy);
         }
     }

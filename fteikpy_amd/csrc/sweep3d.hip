@@ -39,6 +39,7 @@
 #include "history.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -50,7 +51,7 @@ constexpr int kLanesC = 8;      // brick width along y (oriented c)
 constexpr int kSkew = kLanesB + kLanesC - 2;  // steps between first and last lane = 14
 constexpr int kPlane = (kLanesB + 1) * (kLanesC + 1);  // 81 entries: node or cell block
 constexpr int kPrefetch = 1;    // planes resident in LDS ahead of lane (0,0)
-constexpr int kDepth = 4;       // further planes in flight in registers (loop unrolled by it)
+constexpr int kDepth = 2;       // further planes in flight in registers (loop unrolled by it)
 // a slot is rewritten at the end of step s for plane s+kPrefetch; its previous plane was
 // last read (as the upwind plane of lane (7,7)) at step s-1, before that step's barrier
 constexpr int kRing = kPrefetch + kSkew + 2;
@@ -144,8 +145,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
     constexpr int kThreads = 64 * W;
     constexpr int kT = kPlane * W;                           // node entries (W blocks)
     constexpr int NT = (kT + kThreads - 1) / kThreads;       // = 2
-    constexpr int NS = (kPlane + kThreads - 1) / kThreads;   // 2 for W = 1, else 1
-    constexpr int kPitch = dpitch(kT + kPlane);
+    constexpr int kPitch = dpitch(kT);
     constexpr int kCPitch = bpitch(kT);
     __shared__ double ring[kRing][kPitch];
     __shared__ uint8_t ringc[kRing][kCPitch];
@@ -244,29 +244,16 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
             }
         }
     }
-    long long s_off[NS];
-    bool s_use[NS];
-#pragma unroll
-    for (int m = 0; m < NS; ++m) {
-        const int e = tid + kThreads * m;
-        s_use[m] = e < kPlane;
-        const int r = s_use[m] ? e : 0;
-        const int cb = min(max(b0 - 1 + r / 9, 0), A.nx - 1);
-        const int ccl = min(max(c0 - 1 + r % 9, 0), A.ny - 1);
-        s_off[m] = (long long)cell_j(cb) * A.ny + cell_k(ccl);
-    }
 
     // plane p of the brick (a = a_first + p; p = -1 is the upwind plane) lives in slot
     // (p+1) % kRing
     struct Stage {
         double t[NT];
         unsigned c[NT];
-        double s[NS];
     };
     auto load_plane = [&](int p, Stage &v) {
         const int a = a_first + p;
         const long long tb = (long long)node_i(a) * plane_nodes;
-        const long long sb = (long long)cell_i(min(a, A.nz - 1)) * plane_cells;
 #pragma unroll
         for (int n = 0; n < NT; ++n) {
             v.t[n] = kBig;
@@ -276,9 +263,6 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
                 v.c[n] = A.chg[t_off[n] + tb];
             }
         }
-#pragma unroll
-        for (int m = 0; m < NS; ++m)
-            if (s_use[m]) v.s[m] = A.slow[s_off[m] + sb];
     };
     auto store_plane = [&](int p, const Stage &v) {
         double *slot = ring[(p + 1) % kRing];
@@ -289,9 +273,6 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
                 slot[tid + kThreads * n] = v.t[n];
                 cslot[tid + kThreads * n] = (uint8_t)v.c[n];
             }
-#pragma unroll
-        for (int m = 0; m < NS; ++m)
-            if (s_use[m]) slot[kT + tid + kThreads * m] = v.s[m];
     };
 
     for (int p = -1; p < min(kPrefetch, na); ++p) {
@@ -317,7 +298,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
     uint8_t *cg = A.chg + (long long)src * A.node_stride;
 
     const Stencil3 C = stencil_in_vgprs(A);
-    const int tblk = wave * kPlane, sblk = kT;
+    const int tblk = wave * kPlane;
     const int self = (bb + 1) * 9 + (cc + 1);  // this lane's node in a block
     const int up_b = bb * 9 + (cc + 1);        // b-1
     const int up_c = (bb + 1) * 9 + cc;        // c-1
@@ -344,6 +325,33 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
         if (mi == tile_ilo) face_zlo |= c0b;
         if (mi == tile_ihi) face_zhi |= c0b;
     }
+    // Slowness: each lane streams the 4 cells (b-1 | b') x (c-1 | c') around its column (' =
+    // clamped to the grid's last cell) on the cell plane a' = min(a, nz - 1) of its current
+    // node from global memory, one step ahead; the previous step's 4 are the upwind plane's
+    // (a - 1 = the previous a').  Read-only data shared by every source, it stays in cache,
+    // and keeping it out of the LDS ring lets 3 workgroups' waves share a SIMD.  Offsets
+    // within a cell plane fit 32 bits (checked on the host).
+    int scol[4];
+    {
+        const int bm = min(max(b - 1, 0), A.nx - 1), bp = min(max(b, 0), A.nx - 1);
+        const int cm = min(max(c - 1, 0), A.ny - 1), cp = min(max(c, 0), A.ny - 1);
+        scol[0] = cell_j(bm) * A.ny + cell_k(cm);
+        scol[1] = cell_j(bm) * A.ny + cell_k(cp);
+        scol[2] = cell_j(bp) * A.ny + cell_k(cm);
+        scol[3] = cell_j(bp) * A.ny + cell_k(cp);
+    }
+    auto load_cells = [&](int p, double (&v)[4]) {
+        if (inb) {
+            const int pc = min(max(p, -1), na - 1);
+            const long long pb = (long long)cell_i(min(a_first + pc, A.nz - 1)) * plane_cells;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = A.slow[pb + scol[q]];
+        }
+    };
+    double sp[4] = {0.0, 0.0, 0.0, 0.0}, sc[4] = {0.0, 0.0, 0.0, 0.0};
+    load_cells(-bb - cc - 1, sp);
+    load_cells(-bb - cc, sc);
+
     const int nsteps = na + kSkew;
     // ring slots advance by one per step: this lane's current plane p = s - bb - cc sits in
     // slot (p+1) mod kRing, the plane written back at step s in slot (s-kSkew+1) mod kRing
@@ -370,8 +378,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
                     const double tnew = node_update(
                         C, t0, prv[tblk + self], cur[tblk + up_b], cur[tblk + up_c],
                         prv[tblk + up_b], cur[tblk + up_bc], prv[tblk + up_c], prv[tblk + up_bc],
-                        prv[sblk + up_bc], prv[sblk + up_b], prv[sblk + up_c], prv[sblk + self],
-                        cur[sblk + up_bc], cur[sblk + up_b], cur[sblk + up_c], &which);
+                        sp[0], sp[1], sp[2], sp[3], sc[0], sc[1], sc[2], &which);
                     cur[tblk + self] = tnew;
                     if (tnew != t0) {
                         changed_now = true;
@@ -437,6 +444,10 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
             }
             cslot = cslot == kRing - 1 ? 0 : cslot + 1;
             wslot = wslot == kRing - 1 ? 0 : wslot + 1;
+            // this plane's cells become the upwind plane's; fetch the next plane's
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sp[q] = sc[q];
+            load_cells(p + 1, sc);
 
             // plane s+kPrefetch leaves the registers for its slot, whose previous plane was
             // last read one step ago; its register set takes the plane kDepth further on
@@ -1099,6 +1110,7 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
         return FTEIK_ERR_ARG;
     if (want_grad && !sgn) return FTEIK_ERR_ARG;
     if (!(dz > 0.0) || !(dx > 0.0) || !(dy > 0.0)) return FTEIK_ERR_ARG;
+    if ((long long)nx * ny > INT_MAX) return FTEIK_ERR_ARG;  // cell-plane offsets are 32-bit
     if (workspace_bytes < fteik3d_workspace_size(nz, nx, ny, nsrc)) return FTEIK_ERR_WORKSPACE;
     if (nsweep <= 0 && max_sets < 1) return FTEIK_ERR_ARG;
     // defaults measured on MI355X (160x320x320, 256 sources): 2 sources per workgroup
@@ -1441,7 +1453,7 @@ int batch3_prepare(const fteik_problem *P, int n, int nsweep, int max_sets, int 
         if (q.ndim != 3) continue;
         if (q.nz < 1 || q.nx < 1 || q.ny < 1 || q.nsrc < 1 || !q.slow || !q.src || !q.tt ||
             !q.vzero || !q.sets_done || !(q.dz > 0.0) || !(q.dx > 0.0) || !(q.dy > 0.0) ||
-            (want_grad && !q.sgn)) {
+            (want_grad && !q.sgn) || (long long)q.nx * q.ny > INT_MAX) {
             delete b;
             return FTEIK_ERR_ARG;
         }
