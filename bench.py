@@ -27,8 +27,8 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_NSU = 24  # FP64 tt read + tt write + slowness read per node update
 
 # HBM bytes per launch of the dominant kernel, sweep3d_diag_kernel<false, false, 2>, on the
-# default workload (160 x 320 x 320, 256 sources, the library's brick policy, 2 sources per
-# workgroup): rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script, averaged
+# default workload (160 x 320 x 320, 256 sources, the library's default brick height of 24
+# planes, 2 sources per workgroup): rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script, averaged
 # over the 26792 launches of one converged solve and scaled by a 1 GiB device copy
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Commands and figures:
 # profiles/sweep3d_traffic__inclusions_160x320x320_256src.md
@@ -90,7 +90,7 @@ def main():
     with torch.cuda.stream(stream):
         if not args.no_tune and not args.tile:
             # brick height and group for this workload, timed on it before the warm-up
-            tune = fteikpy_amd.autotune.tune3d(eik, src, tiles=(16, 32, 48, -1, 0),
+            tune = fteikpy_amd.autotune.tune3d(eik, src, tiles=(16, 32, 0),
                                                use_graph=use_graph)
         if tune:
             log(f"autotune: tile_a {tune['tile_a']} group {tune['group']} "
@@ -144,26 +144,14 @@ def main():
     achieved = alg_bytes / gpu_s / 1e9
     value = nsrc * args.steps / wall
     # launches of the sweep kernel in the timed region: every set of a batch launches one
-    # kernel per (pass, brick anti-diagonal), until its last source converged; with the
-    # library's policy (tile 0) a batch sweeps whole columns in its first set, then
-    # 32-plane bricks
+    # kernel per (pass, brick anti-diagonal), until its last source converged; brick
+    # height 0 is the library default (24 planes from 32 sources, 16 from 16, else 8)
     tile = args.tile or (tune["tile_a"] if tune else 0)
     group = tune["group"] if tune else (2 if nsrc >= 8 else 1)
-
-    def ndiag(t):
-        t = min(t, nz + 1)
-        return -(-(nz + 1) // t) + -(-(nx + 1) // 8) + -(-(ny + 1) // 8) - 2
-
-    per_solve = 0
-    for s in range(int(k_sets.max())):
-        if tile > 0:
-            t = tile
-        elif nsrc < 32:
-            t = 8
-        else:
-            t = nz + 1 if s == 0 else 32
-        per_solve += 8 * ndiag(t)
-    launches = args.steps * per_solve
+    t = tile if tile > 0 else (24 if nsrc >= 32 else (16 if nsrc >= 16 else 8))
+    t = min(t, nz + 1)
+    ndiag = -(-(nz + 1) // t) + -(-(nx + 1) // 8) + -(-(ny + 1) // 8) - 2
+    launches = args.steps * int(k_sets.max()) * 8 * ndiag
     traffic = None
     if (tuple(shape) == PMC_TRAFFIC["grid"] and nsrc == PMC_TRAFFIC["sources"]
             and tile == PMC_TRAFFIC["tile_a"] and group == PMC_TRAFFIC["group"]):

@@ -1,15 +1,16 @@
 """Autotuner for the 3D sweep: brick height (``tile_a``) and sources per workgroup (``group``).
 
 The batched 3D solve cuts the grid into bricks of ``tile_a`` z planes x 8 x 8 nodes, one
-64-lane wave per brick and source, ``group`` sources per workgroup sharing the staged
-slowness.  The best pair depends on the workload:
+64-lane wave per brick and source, ``group`` sources per workgroup.  The best pair depends
+on the workload:
 
 - tall bricks lose least to the 14-step lane skew in the dense first sweep sets, short
   bricks skip more of the sparse later sets (a brick is skipped only when none of its
   nodes or upwind neighbours changed in the last 8 passes);
 - a workgroup's waves meet at every step, so wide groups wait for their slowest source;
 - a single source is latency bound and prefers short bricks;
-- the library's own policy (brick height 0) changes the height between sweep sets.
+- brick height 0 is the library default for the number of sources (24 planes from 32
+  sources, 16 from 16, else 8).
 
 ``tune3d`` times converged solves of the caller's own workload for a list of candidates
 (first the brick heights at a fixed group, then the groups at the best height) and records
@@ -25,7 +26,7 @@ import numpy as np
 
 _CHOICES = {}
 
-TILES = (8, 16, 32, 48, 64, -1, 0)  # -1: the whole z column; 0: the library's policy
+TILES = (8, 16, 32, 48, -1, 0)  # -1: the whole z column; 0: the library default
 GROUPS = (1, 2, 4)
 
 
@@ -60,8 +61,8 @@ def load(path):
 
 def default3d(nsrc):
     """The library's defaults for nsrc sources (fteik3d_solve with tile_a = group = 0):
-    brick height 0 = its per-phase policy (from 32 sources on: whole columns in the first
-    set, 32 planes after it; a fixed 8 below 32 sources)."""
+    brick height 0 = 24 planes from 32 sources, 16 from 16, else 8; 2 sources per
+    workgroup from 8 sources."""
     return 0, (2 if nsrc >= 8 else 1)
 
 

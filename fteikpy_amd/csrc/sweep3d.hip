@@ -642,43 +642,6 @@ __global__ void zero_bytes_kernel(uint8_t *p, long long n) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
 }
 
-// Tile bytes of another brick height from the per-node change bytes (the OR of a tile's
-// node bytes), for the sources still changing: one wave per (tile, listed source).
-__global__ void tsum_rebuild_kernel(const uint8_t *chg, uint8_t *tsum, uint64_t *faces,
-                                    const int *list, const int *nlist, long long node_stride,
-                                    int NZ, int NX, int NY, int tile_a, int nbb, int nbc,
-                                    int ntiles) {
-    const int tile = blockIdx.x, li = blockIdx.y;
-    if (li >= *nlist) return;
-    const int src = list[li];
-    const int tk = tile % nbc, tj = (tile / nbc) % nbb, ti = tile / (nbc * nbb);
-    const int i0 = ti * tile_a, i1 = min(NZ, i0 + tile_a);
-    const int lane = threadIdx.x;
-    const int j = tj * 8 + (lane >> 3), k = tk * 8 + (lane & 7);
-    unsigned acc = 0, zlo = 0, zhi = 0;
-    if (j < NX && k < NY) {
-        const uint8_t *c = chg + (long long)src * node_stride + (long long)j * NY + k;
-        for (int i = i0; i < i1; ++i) acc |= c[(long long)i * NX * NY];
-        zlo = c[(long long)i0 * NX * NY];
-        zhi = c[(long long)(i1 - 1) * NX * NY];
-    }
-    unsigned f[7] = {acc, zlo, zhi, j == tj * 8 ? acc : 0u, j == min(NX, tj * 8 + 8) - 1 ? acc : 0u,
-                     k == tk * 8 ? acc : 0u, k == min(NY, tk * 8 + 8) - 1 ? acc : 0u};
-    if (!(j < NX && k < NY)) f[3] = f[4] = f[5] = f[6] = 0u;
-#pragma unroll
-    for (int q = 0; q < 7; ++q)
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) f[q] |= __shfl_xor(f[q], o);
-    if (lane == 0) {
-        tsum[(long long)src * ntiles + tile] = (uint8_t)f[0];
-        if (faces) {
-            unsigned long long packed = 0;
-            for (int q = 0; q < 6; ++q) packed |= (unsigned long long)(f[q + 1] & 0xffu) << (8 * q);
-            faces[(long long)src * ntiles + tile] = packed;
-        }
-    }
-}
-
 __device__ __forceinline__ double t_ana3(int i, int j, int k, double dz, double dx, double dy,
                                          const SrcParam &s, double vzero) {
     const double a = dz * ((double)i - s.zsa), b = dx * ((double)j - s.xsa),
@@ -866,13 +829,11 @@ size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 // Captured sweep sets (one per brick height), destroyed on every return path.
 struct Graphs {
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[1] = {nullptr};
+    hipGraphExec_t exec[1] = {nullptr};
     ~Graphs() {
-        for (int g = 0; g < 2; ++g) {
-            if (exec[g]) (void)hipGraphExecDestroy(exec[g]);
-            if (graph[g]) (void)hipGraphDestroy(graph[g]);
-        }
+        if (exec[0]) (void)hipGraphExecDestroy(exec[0]);
+        if (graph[0]) (void)hipGraphDestroy(graph[0]);
     }
 };
 
@@ -1020,10 +981,10 @@ int launch_seed3d(const SweepArgs &a, const Ws3 &w, const double *slow, int nsrc
     return hip_status(hipGetLastError());
 }
 
-// brick height default: 32-plane bricks for batches (shorter bricks skip more of the sparse
-// later sets, taller ones waste less on the 14-step skew), 8 planes for a few sources
-// (latency bound).  Measured on MI355X.
-int default_tile(int nsrc) { return nsrc >= 32 ? 32 : 8; }
+// brick height default: 24-plane bricks for batches (shorter bricks are skipped more often
+// in the sparse later sets, taller ones waste less on the 14-step skew), 16 from 16
+// sources, 8 planes for a few sources (latency bound).  Measured on MI355X.
+int default_tile(int nsrc) { return nsrc >= 32 ? 24 : (nsrc >= 16 ? 16 : 8); }
 
 // Sweep arguments of one model and its workspace carve-up (the history fields are
 // cleared; the logged solve sets them).
@@ -1131,37 +1092,19 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
 
     const int NZ = nz + 1, NX = nx + 1, NY = ny + 1;
     const long long nn = (long long)NZ * NX * NY;
-    // Brick height per set.  A height given by the caller is used throughout.  Otherwise,
-    // for batches (>= 32 sources): whole columns in the first set (every node is evaluated;
-    // the lane skew costs least), then 32 planes (shorter bricks are skipped more often in
-    // the sparse later sets) -- measured on MI355X (an 8-plane phase for the last few
-    // sources measured slower).  The tile bytes of the second height are rebuilt from the
-    // per-node change bytes at the switch.
-    int geo_tile[2];
-    const bool adaptive = tile_a <= 0 && nsrc >= 32 && schedule == 1;
-    if (adaptive) {
-        geo_tile[0] = NZ;
-        geo_tile[1] = default_tile(nsrc);
-    } else {
-        geo_tile[0] = geo_tile[1] = tile_a > 0 ? tile_a : default_tile(nsrc);
-    }
+    // brick height: the caller's, else the library default for this many sources
     Ws3 w;
-    SweepArgs geo[2];
-    std::vector<int> geo_nb[2];
-    for (int g = 0; g < 2; ++g) {
-        make_args(slow, nz, nx, ny, dz, dx, dy, nsrc, geo_tile[g], group,
-                  want_grad ? sgn : nullptr, tt, sets_done, (char *)workspace, &geo[g], &w);
-        geo[g].log_node = hv ? hv->node : nullptr;
-        geo[g].log_old = hv ? hv->old : nullptr;
-        geo[g].log_count = hv ? &hv->hdr->count : nullptr;
-        geo[g].log_cap = hv ? hcap : 0;
-        geo_nb[g].resize(geo[g].ndiag);
-        for (int d = 0; d < geo[g].ndiag; ++d)
-            geo_nb[g][d] = bricks_on_diag(d, geo[g].nba, geo[g].nbb, geo[g].nbc);
-    }
-    SweepArgs &a = geo[0];
+    SweepArgs a;
+    make_args(slow, nz, nx, ny, dz, dx, dy, nsrc, tile_a > 0 ? tile_a : default_tile(nsrc), group,
+              want_grad ? sgn : nullptr, tt, sets_done, (char *)workspace, &a, &w);
+    a.log_node = hv ? hv->node : nullptr;
+    a.log_old = hv ? hv->old : nullptr;
+    a.log_count = hv ? &hv->hdr->count : nullptr;
+    a.log_cap = hv ? hcap : 0;
+    std::vector<int> nb(a.ndiag);
+    for (int d = 0; d < a.ndiag; ++d) nb[d] = bricks_on_diag(d, a.nba, a.nbb, a.nbc);
     std::vector<int> cum(a.ndiag + 1, 0);
-    for (int d = 0; d < a.ndiag; ++d) cum[d + 1] = cum[d] + geo_nb[0][d];
+    for (int d = 0; d < a.ndiag; ++d) cum[d + 1] = cum[d] + nb[d];
     const long long gmax = a.gmax;
     if (8LL * a.ntiles * gmax >= (1LL << 31)) return FTEIK_ERR_ARG;  // task index is an int
     const int pblocks = schedule == 0 ? pers_blocks(want_grad != 0, group) : 0;
@@ -1208,9 +1151,7 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
         }
     }
 
-    auto enqueue_set = [&](int g) -> int {
-        SweepArgs &a = geo[g];
-        const std::vector<int> &nb = geo_nb[g];
+    auto enqueue_set = [&]() -> int {
         if (schedule == 0) {
             const long long np = (long long)a.ntiles * gmax;
             hipLaunchKernelGGL(set_reset_kernel,
@@ -1240,48 +1181,30 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
         return hip_status(hipGetLastError());
     };
 
-    // one captured sweep set per brick height, instantiated on first use
+    // the sweep set captured once, instantiated on first use
     Graphs gr;
-    auto run_set = [&](int g) -> int {
+    auto run_set = [&]() -> int {
         if (use_graph && stream) {  // the legacy default stream cannot be captured
-            if (!gr.exec[g]) {
+            if (!gr.exec[0]) {
                 FTEIK_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-                int st = enqueue_set(g);
-                hipError_t e = hipStreamEndCapture(stream, &gr.graph[g]);
+                int st = enqueue_set();
+                hipError_t e = hipStreamEndCapture(stream, &gr.graph[0]);
                 if (st) return st;
                 FTEIK_CHECK(e);
-                FTEIK_CHECK(hipGraphInstantiate(&gr.exec[g], gr.graph[g], nullptr, nullptr, 0));
+                FTEIK_CHECK(hipGraphInstantiate(&gr.exec[0], gr.graph[0], nullptr, nullptr, 0));
             }
-            return hip_status(hipGraphLaunch(gr.exec[g], stream));
+            return hip_status(hipGraphLaunch(gr.exec[0], stream));
         }
-        return enqueue_set(g);
-    };
-    int cur = 0;  // brick height of the previous set
-    auto switch_to = [&](int g) -> int {
-        if (geo[g].tile_a == geo[cur].tile_a) {
-            cur = g;
-            return 0;
-        }
-        const SweepArgs &b = geo[g];
-        hipLaunchKernelGGL(tsum_rebuild_kernel, dim3((unsigned)b.ntiles, (unsigned)nsrc), dim3(64),
-                           0, stream, (const uint8_t *)w.chg, w.tsum, b.faces, (const int *)w.list,
-                           (const int *)w.nlist, nn, NZ, NX, NY, b.tile_a, b.nbb, b.nbc,
-                           b.ntiles);
-        cur = g;
-        return hip_status(hipGetLastError());
+        return enqueue_set();
     };
 
     int status = 0;
     if (nsweep > 0) {
-        for (int s = 0; s < nsweep && !status; ++s) {
-            status = switch_to(s == 0 ? 0 : 1);
-            if (!status) status = run_set(cur);
-        }
+        for (int s = 0; s < nsweep && !status; ++s) status = run_set();
     } else {
         int nact = nsrc;
         for (int s = 0; s < max_sets && nact > 0 && !status; ++s) {
-            status = switch_to(s == 0 ? 0 : 1);
-            if (!status) status = run_set(cur);
+            status = run_set();
             if (status) break;
             hipError_t e = hipMemcpyAsync(&nact, w.nlist, sizeof(int), hipMemcpyDeviceToHost, stream);
             if (e == hipSuccess) e = hipStreamSynchronize(stream);

@@ -20,7 +20,7 @@ def test_tune3d_records_choice_and_result_is_unchanged(device):
     autotune.clear()
     res = autotune.tune3d(eik, src, tiles=(4, 8, 0), groups=(1, 2))
     print("tune3d", res, flush=True)
-    assert res["tile_a"] in (0, 4, 8) and res["group"] in (1, 2)  # 0: the library policy
+    assert res["tile_a"] in (0, 4, 8) and res["group"] in (1, 2)  # 0: the library default
     assert len(res["timings"]) >= 3
     assert autotune.lookup3d(shape, len(src)) == (res["tile_a"], res["group"])
     grids = eik.solve(src, nsweep=None)

@@ -229,24 +229,26 @@ def test_many_sets_skip_bitwise(fa):
                                    rtol=1e-13, atol=1e-15)
 
 
-def test_brick_height_policy_bitwise(fa):
-    """From 32 sources on, the default brick height changes after the first set (whole
-    columns, then 32 planes) with the tile bytes rebuilt at the switch: every intermediate
-    nsweep and the converged grids stay bit-identical to the oracle."""
-    shape, d = (45, 27, 30), (9.0, 8.0, 10.0)  # 46 node planes: whole column != 32
+def test_default_brick_heights_bitwise(fa):
+    """The library's default brick heights (24 planes from 32 sources, 16 from 16) on a grid
+    whose 46 node planes leave a partial last brick: every intermediate nsweep and the
+    converged grids stay bit-identical to the oracle."""
+    shape, d = (45, 27, 30), (9.0, 8.0, 10.0)
     v = _rough_model(shape, 17)
     rng = np.random.default_rng(17)
-    srcs = rng.random((40, 3)) * np.array(shape) * d
     eik = fa.Eikonal3D(v, d)
-    ref, _, _, kref = oracle.solve3d(1.0 / v, *d, srcs, nsweep=0)
-    assert kref.max() > kref.min()  # sources retire at different sets
-    for use_graph in (True, False):
-        tt, _, _, sets = eik.solve_tensors(srcs, nsweep=None, use_graph=use_graph)
-        assert list(sets) == list(kref)
-        assert np.array_equal(tt.cpu().numpy(), ref)
-    for ns in (1, 2, 3):
-        r = oracle.solve3d(1.0 / v, *d, srcs, nsweep=ns)[0]
-        assert np.array_equal(eik.solve_tensors(srcs, nsweep=ns, use_graph=True)[0].cpu().numpy(), r)
+    for n in (40, 16):
+        srcs = rng.random((n, 3)) * np.array(shape) * d
+        ref, _, _, kref = oracle.solve3d(1.0 / v, *d, srcs, nsweep=0)
+        assert kref.max() > kref.min()  # sources retire at different sets
+        for use_graph in (True, False):
+            tt, _, _, sets = eik.solve_tensors(srcs, nsweep=None, use_graph=use_graph)
+            assert list(sets) == list(kref)
+            assert np.array_equal(tt.cpu().numpy(), ref)
+        for ns in (1, 2, 3):
+            r = oracle.solve3d(1.0 / v, *d, srcs, nsweep=ns)[0]
+            tt = eik.solve_tensors(srcs, nsweep=ns, use_graph=True)[0]
+            assert np.array_equal(tt.cpu().numpy(), r)
 
 
 @pytest.mark.parametrize("tile_a, group", [(8, 2), (16, 1), (0, 0)])
