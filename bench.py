@@ -29,11 +29,11 @@ BYTES_PER_NSU = 24  # FP64 tt read + tt write + slowness read per node update
 # HBM bytes per launch of the dominant kernel, sweep3d_diag_kernel<false, false, 2>, on the
 # default workload (160 x 320 x 320, 256 sources, the library's default brick height of 24
 # planes, 2 sources per workgroup): rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script, averaged
-# over the 26792 launches of one converged solve and scaled by a 1 GiB device copy
+# over the 27144 launches of one converged solve and scaled by a 1 GiB device copy
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Commands and figures:
 # profiles/sweep3d_traffic__inclusions_160x320x320_256src.md
 PMC_TRAFFIC = {"grid": (160, 320, 320), "sources": 256, "tile_a": 0, "group": 2,
-               "fetch_bytes": 169255.91 * 2.0 * 1024, "write_bytes": 8454.37 * 1.0 * 1024}
+               "fetch_bytes": 151243.78 * 2.0 * 1024, "write_bytes": 8384.98 * 1.0 * 1024}
 
 
 def main():
