@@ -37,6 +37,44 @@ struct SrcParam {
     int iflag;             // 2D source classification (1, 2 or 3); unused in 3D
 };
 
+// a / b, correctly rounded, for a divisor b fixed per solve with y = 1 / b correctly
+// rounded.  q0 = a y is within 2 ulp of a / b; one correction with the exact remainder
+// a - b q0 (an FMA) brings it within 1 ulp, and by Markstein's theorem (y within half an
+// ulp of 1 / b, q within 1 ulp of a / b) a second one, q + (a - b q) y rounded once, is the
+// correctly rounded quotient.  Five dependent operations instead of the generic division's
+// scaled reciprocal refinement.  Valid away from overflow and the subnormal range, which
+// traveltimes and inverse squared spacings stay far from.  An infinite a gives NaN where
+// a / b gives +-inf (the caller must not depend on the difference).
+__device__ __forceinline__ double div_fixed(double a, double b, double y) {
+    const double q0 = a * y;
+    const double q1 = fma(fma(-b, q0, a), y, q0);
+    return fma(fma(-b, q1, a), y, q1);
+}
+
+// sqrt(x), correctly rounded, with the compiler's own refinement sequence for doubles
+// without its range handling: for x in [2^-767, +inf) the library first scales only
+// below 2^-767 and special-cases only 0 and +inf, so there this is the same sequence on
+// the same value (rsq estimate, then the Goldschmidt/Newton steps) and returns the same
+// double.  Any other argument (0, tiny, infinite, negative, NaN) takes the library sqrt; the
+// branch is uniform over the wave, and the node update produces such arguments only as
+// exact zeros.
+__device__ __forceinline__ double sqrt_rn(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    const bool other = !(x >= 0x1p-767 && x < __builtin_inf());
+    if (__builtin_expect(__any(other), 0)) {
+        if (other) g = sqrt(x);
+    }
+    return g;
+}
+
 // Converts a HIP error into the library's negative return code.
 inline int hip_status(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 

@@ -32,19 +32,6 @@ inline Stencil3 make_stencil3(double dz, double dx, double dy) {
     return c;
 }
 
-// a / b, correctly rounded, for a divisor b fixed per solve with y = 1 / b correctly
-// rounded.  q0 = a y is within 2 ulp of a / b; one correction with the exact remainder
-// a - b q0 (an FMA) brings it within 1 ulp, and by Markstein's theorem (y within half an
-// ulp of 1 / b, q within 1 ulp of a / b) a second one, q + (a - b q) y rounded once, is the
-// correctly rounded quotient.  Five dependent operations instead of the generic division's
-// scaled reciprocal refinement.  Valid away from overflow and the subnormal range, which
-// traveltimes and inverse squared spacings stay far from.
-__device__ __forceinline__ double div_fixed(double a, double b, double y) {
-    const double q0 = a * y;
-    const double q1 = fma(fma(-b, q0, a), y, q0);
-    return fma(fma(-b, q1, a), y, q1);
-}
-
 // A copy of the constants held in vector registers for the whole sweep loop: as uniform
 // values they would take ~34 scalar registers (the loop's bookkeeping already fills the
 // scalar file) and, with one scalar operand allowed per vector instruction, extra moves.
@@ -58,30 +45,6 @@ __device__ __forceinline__ Stencil3 stencil_in_vgprs(const Stencil3 &a) {
     pin_vgpr(c.szx); pin_vgpr(c.szy); pin_vgpr(c.sxy);
     pin_vgpr(c.rzx); pin_vgpr(c.rzy); pin_vgpr(c.rxy); pin_vgpr(c.rsum);
     return c;
-}
-
-// sqrt(x), correctly rounded, with the compiler's own refinement sequence for doubles
-// without its range handling: for x in [2^-767, +inf) the library first scales only
-// below 2^-767 and special-cases only 0 and +inf, so there this is the same sequence on
-// the same value (rsq estimate, then the Goldschmidt/Newton steps) and returns the same
-// double.  Any other argument (0, tiny, infinite, negative, NaN) takes the library sqrt; the
-// branch is uniform over the wave, and the node update produces such arguments only as
-// exact zeros.
-__device__ __forceinline__ double sqrt_rn(double x) {
-    const double y = __builtin_amdgcn_rsq(x);
-    double g = x * y, h = y * 0.5;
-    const double r = fma(-h, g, 0.5);
-    g = fma(g, r, g);
-    h = fma(h, r, h);
-    double d = fma(-g, g, x);
-    g = fma(d, h, g);
-    d = fma(-g, g, x);
-    g = fma(d, h, g);
-    const bool other = !(x >= 0x1p-767 && x < __builtin_inf());
-    if (__builtin_expect(__any(other), 0)) {
-        if (other) g = sqrt(x);
-    }
-    return g;
 }
 
 __device__ __forceinline__ double min4(double a, double b, double c, double d) {
@@ -103,7 +66,9 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
     const double dz = A.dz, dx = A.dx, dy = A.dy;
     const double dz2i = A.dz2i, dx2i = A.dx2i, dy2i = A.dy2i;
     // the reference's (dz2i + dx2i), dz2i * dx2i and x / (dz2i + dx2i) are evaluated from
-    // the precomputed sums, products and reciprocals: the same doubles
+    // the precomputed sums, products and reciprocals: the same doubles.  A quotient that
+    // would be infinite (infinite slowness) comes out NaN instead; both lose every fmin
+    // below (t0 is finite), so the unchecked division suffices here.
 
     // 1D operators (refracted times), _fteik3d.py:78-104
     const double t1d1 = tv + dz * min4(s000, s001, s010, s011);
