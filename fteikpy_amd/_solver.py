@@ -5,7 +5,7 @@ for a batch of sources).  Keyword-only extensions:
 
 - ``nsweep=None`` (or <= 0): sweep until a set changes no node, at most ``max_sets`` sets;
 - ``tile_a``: z-extent of the 3D sweep bricks (0 = library default);
-- ``group``: sources per workgroup sharing the staged slowness (0 = default; 1, 2, 4, 8);
+- ``group``: sources per workgroup (0 = default; 1, 2, 4, 8);
   with both left at 0, a choice recorded by ``fteikpy_amd.autotune.tune3d`` for this grid
   shape and number of sources is used;
 - ``schedule``: "diagonal" (one launch per brick anti-diagonal and pass; the default) or

@@ -12,10 +12,11 @@
 // coordinates; in each pass a tile is processed as one brick by one 64-lane wave per
 // source.  Lane (bb, cc) owns column (x, y) = (b0+bb, c0+cc) of the pass's oriented
 // coordinates and, at step s, works on plane p = s - bb - cc: all lanes of a step lie on one
-// node hyperplane, so every upwind value was produced at an earlier step.  z planes are
-// staged through an LDS ring with coalesced global loads (4 planes in flight in registers)
-// and written back coalesced once every lane is past them.  W waves of a workgroup run the
-// same brick for W sources and share the staged slowness.
+// node hyperplane, so every upwind value was produced at an earlier step.  Traveltimes and
+// change bytes of z planes are staged through an LDS ring with coalesced global loads (2
+// planes in flight in registers) and written back coalesced once every lane is past them;
+// each lane reads the slowness cells around its node from global memory (cached, shared by
+// all sources).  W waves of a workgroup run the same brick for W sources.
 //
 // Skipping.  Per node, one byte records in which of the last 8 passes the node changed; a
 // node whose 7 upwind neighbours all read zero is skipped (its candidates equal those of

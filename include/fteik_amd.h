@@ -50,13 +50,14 @@ size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc);
  * Replaces solve3d / fteik3d / fteik3d_vectorized (fteikpy/_fteik/_fteik3d.py:394-525),
  * called from Eikonal3D.solve (fteikpy/_solver.py:120-126).
  *
- * slow      device [nz, nx, ny] slowness (1/velocity), shared by all sources
+ * slow      device [nz, nx, ny] slowness (1/velocity), shared by all sources; one cell
+ *           plane (nx * ny) must stay below 2^31 cells (else FTEIK_ERR_ARG)
  * src       host   [nsrc, 3] source coordinates relative to the grid origin
  * nsweep    > 0: exactly nsweep sweep sets (the reference's nsweep);
  *           <= 0: sets until one changes no node, at most max_sets
  * want_grad record the upwind direction of every update (needed by fteik3d_gradient)
  * tile_a    nodes per brick along z (0 = library default for the shape)
- * group     sources per workgroup sharing the staged slowness (0 = default; 1, 2, 4, 8)
+ * group     sources per workgroup (0 = default; 1, 2, 4, 8)
  * schedule  1: one launch per (pass, brick anti-diagonal), ordered by the stream;
  *           0: persistent (one launch per sweep set, tasks wait on per-tile progress
  *           counters) -- currently refused with FTEIK_ERR_UNSUPPORTED

@@ -228,3 +228,17 @@ def test_autotune_records_and_persists(tmp_path):
     autotune.load(path)
     assert autotune.lookup3d((10, 20, 30), 4) == (16, 2)
     autotune.clear()
+
+
+def test_cell_plane_limit():
+    """In-plane cell offsets of the 3D sweep are 32-bit: a cell plane of more than 2^31 - 1
+    cells is refused from the host, before any device work (fake pointers)."""
+    from fteikpy_amd import _lib
+
+    L = _lib.lib()
+    fake = ctypes.c_void_p(0x1000)
+    src = np.array([0.5, 0.5, 0.5])
+    sp = src.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    st = L.fteik3d_solve(fake, 1, 50000, 50000, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, 0, 1, fake,
+                         None, fake, fake, fake, 1 << 62, 0, None)
+    assert st == _lib.ERR_ARG
