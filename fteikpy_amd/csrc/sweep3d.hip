@@ -72,7 +72,8 @@ struct SweepArgs : Stencil3 {
     double *tt;
     uint8_t *sgn;
     uint8_t *chg;           // per node: bit p set if the node changed in the last pass p
-    uint8_t *tsum;          // per source and tile: OR of the tile nodes' change bits
+    uint8_t *tsum;          // per source and tile: bit p set if a node of the tile changed
+                            // in the latest pass p (bits only accumulate in the first set)
     uint64_t *faces;        // per source and tile: byte f = OR of the change bits of the
                             // tile's face f (0/1 first/last node plane, 2/3 first/last x
                             // column, 4/5 first/last y column); null: tile bytes only
@@ -833,13 +834,13 @@ int bricks_on_diag(int d, int nba, int nbb, int nbc) {
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-// Captured sweep sets (one per brick height), destroyed on every return path.
+// A captured sweep set, destroyed on every return path.
 struct Graphs {
-    hipGraph_t graph[1] = {nullptr};
-    hipGraphExec_t exec[1] = {nullptr};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
     ~Graphs() {
-        if (exec[0]) (void)hipGraphExecDestroy(exec[0]);
-        if (graph[0]) (void)hipGraphDestroy(graph[0]);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
     }
 };
 
@@ -1191,15 +1192,15 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
     Graphs gr;
     auto run_set = [&]() -> int {
         if (use_graph && stream) {  // the legacy default stream cannot be captured
-            if (!gr.exec[0]) {
+            if (!gr.exec) {
                 FTEIK_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
                 int st = enqueue_set();
-                hipError_t e = hipStreamEndCapture(stream, &gr.graph[0]);
+                hipError_t e = hipStreamEndCapture(stream, &gr.graph);
                 if (st) return st;
                 FTEIK_CHECK(e);
-                FTEIK_CHECK(hipGraphInstantiate(&gr.exec[0], gr.graph[0], nullptr, nullptr, 0));
+                FTEIK_CHECK(hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0));
             }
-            return hip_status(hipGraphLaunch(gr.exec[0], stream));
+            return hip_status(hipGraphLaunch(gr.exec, stream));
         }
         return enqueue_set();
     };
@@ -1500,13 +1501,13 @@ int batch3_run(Batch3 *b, int use_graph, hipStream_t stream) {
     if (use_graph && stream) {
         FTEIK_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         int st = enqueue_set();
-        hipError_t e = hipStreamEndCapture(stream, &gr.graph[0]);
+        hipError_t e = hipStreamEndCapture(stream, &gr.graph);
         if (st) return st;
         FTEIK_CHECK(e);
-        FTEIK_CHECK(hipGraphInstantiate(&gr.exec[0], gr.graph[0], nullptr, nullptr, 0));
+        FTEIK_CHECK(hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0));
     }
     auto run_set = [&]() -> int {
-        if (gr.exec[0]) return hip_status(hipGraphLaunch(gr.exec[0], stream));
+        if (gr.exec) return hip_status(hipGraphLaunch(gr.exec, stream));
         return enqueue_set();
     };
     int status = 0;
