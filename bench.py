@@ -178,6 +178,9 @@ def main():
                          "need to converge; oracle C restatement of the reference's numba "
                          "path, OpenMP over sources"}
 
+    torch.cuda.empty_cache()
+    copy_gbs = stream_copy(torch, dev)
+    us_launch = gpu_s / launches * 1e6
     res = {
         "metric": "converged FP64 source-grids/sec 3D, batched vs plain GPU; fwd+adjoint s/iter",
         "value": value,
@@ -206,14 +209,18 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": alg_bytes / launches,
-                     "launches": launches, "us_per_launch": gpu_s / launches * 1e6,
+                     "launches": launches, "us_per_launch": us_launch,
+                     "traffic_GBps": traffic / us_launch / 1e3 if traffic else None,
+                     "stream_copy_GBps": copy_gbs, "achieved_over_stream_copy": achieved / copy_gbs,
                      "note": "achieved = algorithmic bytes per launch (24 B per node update x 8 "
                              "passes x nodes x sets executed, over the launches of the timed "
                              "solves) / mean launch time (HIP events on the kernels' stream); "
                              "traffic = measured HBM bytes per launch (rocprofv3 FETCH_SIZE + "
                              "WRITE_SIZE, copy-calibrated; null for other configurations): "
                              "below the algorithmic bytes because inactive bricks and "
-                             "unchanged nodes are skipped"},
+                             "unchanged nodes are skipped; traffic_GBps = traffic / mean launch "
+                             "time; stream_copy_GBps = a 1 GiB device copy (read + write bytes) "
+                             "timed here"},
         "cpu_baseline": cpu,
         "fwd_adjoint": adj,
         "analytic": ana,
@@ -239,6 +246,28 @@ def copy_calibration(gib):
     dt = time.perf_counter() - t0
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "seconds": dt,
                       "checksum": float(y[::4096].sum())}), flush=True)
+
+
+def stream_copy(torch, dev, gib=1, reps=10):
+    """Device-to-device copy bandwidth (STREAM copy): bytes read + written per second over
+    `reps` copies of a `gib` GiB FP64 buffer, timed with HIP events on the copy stream."""
+    n = gib * (1 << 27)
+    a = torch.empty(n, dtype=torch.float64, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    stream = torch.cuda.Stream(device=dev)
+    with torch.cuda.stream(stream):
+        for _ in range(2):
+            b.copy_(a)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            b.copy_(a)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    secs = e0.elapsed_time(e1) / 1e3
+    del a, b
+    return 2.0 * n * 8 * reps / secs / 1e9
 
 
 def analytic_errors(torch, fa, dev):
