@@ -135,7 +135,7 @@ __global__ void fill2d_kernel(double *tt, long long n, uint8_t *sgn) {
 }
 
 // Source initialisation (_fteik2d.py:355-617) and the sweep sets of source s, by one lane.
-__device__ void solve_source2(const double *slow, int nz, int nx, double dz, double dx,
+__device__ __forceinline__ void solve_source2(const double *slow, int nz, int nx, double dz, double dx,
                               const SrcParam &P, int s, int nsweep, int max_sets,
                               double *tt_all, uint8_t *sgn_all, double *vzero_out,
                               int *sets_done) {
@@ -295,7 +295,10 @@ __device__ void solve_source2(const double *slow, int nz, int nx, double dz, dou
     sets_done[s] = done;
 }
 
-__global__ void fteik2d_kernel(const double *slow, int nz, int nx, double dz, double dx,
+// One lane per block (see the launches): the bound lets the compiler keep the whole
+// per-source chain in registers.
+__global__ __launch_bounds__(64) void fteik2d_kernel(const double *slow, int nz, int nx, double dz,
+                                                     double dx,
                                const SrcParam *prm, int nsrc, int nsweep, int max_sets,
                                double *tt_all, uint8_t *sgn_all, double *vzero_out,
                                int *sets_done) {
@@ -317,8 +320,23 @@ struct Prob2 {
     double *vzero;
     int *sets_done;
 };
-__global__ void fteik2d_batch_kernel(const Prob2 *probs, const int *cum, int nmodel, int nsweep,
-                                     int max_sets) {
+// The kernel's view of the same table: every pointer addresses device global memory
+// (address space 1), which lets the inlined chain use global instead of flat instructions.
+#define FTEIK_GLOBAL __attribute__((address_space(1)))
+struct Prob2Dev {
+    const FTEIK_GLOBAL double *slow;
+    int nz, nx;
+    double dz, dx;
+    const FTEIK_GLOBAL SrcParam *prm;
+    FTEIK_GLOBAL double *tt;
+    FTEIK_GLOBAL uint8_t *sgn;
+    FTEIK_GLOBAL double *vzero;
+    FTEIK_GLOBAL int *sets_done;
+};
+static_assert(sizeof(Prob2Dev) == sizeof(Prob2), "same table layout");
+
+__global__ __launch_bounds__(64) void fteik2d_batch_kernel(const Prob2 *probs, const int *cum,
+                                                           int nmodel, int nsweep, int max_sets) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cum[nmodel]) return;
     int lo = 0, hi = nmodel - 1;
@@ -327,10 +345,11 @@ __global__ void fteik2d_batch_kernel(const Prob2 *probs, const int *cum, int nmo
         if (cum[mid] <= t) lo = mid;
         else hi = mid - 1;
     }
-    const Prob2 Q = probs[lo];
+    const Prob2Dev Q = reinterpret_cast<const Prob2Dev *>(probs)[lo];
     const int s = t - cum[lo];
-    solve_source2(Q.slow, Q.nz, Q.nx, Q.dz, Q.dx, Q.prm[s], s, nsweep, max_sets, Q.tt, Q.sgn,
-                  Q.vzero, Q.sets_done);
+    const SrcParam P = ((const SrcParam *)Q.prm)[s];  // read once, before the chain
+    solve_source2((const double *)Q.slow, Q.nz, Q.nx, Q.dz, Q.dx, P, s, nsweep, max_sets,
+                  (double *)Q.tt, (uint8_t *)Q.sgn, (double *)Q.vzero, (int *)Q.sets_done);
 }
 
 // 2D gradient post-pass, _fteik2d.py:623-639 (seeded analytic components survive where
