@@ -227,7 +227,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
     const long long plane_cells = (long long)A.nx * A.ny;
 
     // Staging: node entries e = tid + 64 W n (< 81 W; node (eb, ec) of wave e / 81's
-    // block), cell entries e = tid + 64 W m (< 81).  Nodes outside the grid read as Big.
+    // block), traveltime and change byte.  Nodes outside the grid read as Big.
     long long t_off[NT];
     bool t_use[NT], t_ok[NT];
 #pragma unroll
