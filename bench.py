@@ -27,13 +27,13 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_NSU = 24  # FP64 tt read + tt write + slowness read per node update
 
 # HBM bytes per launch of the dominant kernel, sweep3d_diag_kernel<false, false, 2>, on the
-# default workload (160 x 320 x 320, 256 sources, the library's default brick height of 24
-# planes, 2 sources per workgroup): rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script, averaged
-# over the 27144 launches of one converged solve and scaled by a 1 GiB device copy
+# default workload (160 x 320 x 320, 256 sources, the library's default brick height, 27
+# planes here, 2 sources per workgroup): rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script, averaged
+# over the 26832 launches of one converged solve and scaled by a 1 GiB device copy
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Commands and figures:
 # profiles/sweep3d_traffic__inclusions_160x320x320_256src.md
 PMC_TRAFFIC = {"grid": (160, 320, 320), "sources": 256, "tile_a": 0, "group": 2,
-               "fetch_bytes": 151243.78 * 2.0 * 1024, "write_bytes": 8384.98 * 1.0 * 1024}
+               "fetch_bytes": 159025.98 * 2.0 * 1024, "write_bytes": 8467.68 * 1.0 * 1024}
 
 
 def main():
@@ -146,10 +146,10 @@ def main():
     value = nsrc * args.steps / wall
     # launches of the sweep kernel in the timed region: every set of a batch launches one
     # kernel per (pass, brick anti-diagonal), until its last source converged; brick
-    # height 0 is the library default (24 planes from 32 sources, 16 from 16, else 8)
+    # height 0 is the library default (autotune.default_tile)
     tile = args.tile or (tune["tile_a"] if tune else 0)
     group = tune["group"] if tune else (2 if nsrc >= 8 else 1)
-    t = tile if tile > 0 else (24 if nsrc >= 32 else (16 if nsrc >= 16 else 8))
+    t = tile if tile > 0 else fteikpy_amd.autotune.default_tile(nsrc, nz + 1)
     t = min(t, nz + 1)
     ndiag = -(-(nz + 1) // t) + -(-(nx + 1) // 8) + -(-(ny + 1) // 8) - 2
     launches = args.steps * int(k_sets.max()) * 8 * ndiag

@@ -9,8 +9,8 @@ on the workload:
   nodes or upwind neighbours changed in the last 8 passes);
 - a workgroup's waves meet at every step, so wide groups wait for their slowest source;
 - a single source is latency bound and prefers short bricks;
-- brick height 0 is the library default for the number of sources (24 planes from 32
-  sources, 16 from 16, else 8).
+- brick height 0 is the library default (``default_tile``): from 32 sources the height in
+  20..28 with the fewest steps down a column, 16 planes from 16 sources, else 8.
 
 ``tune3d`` times converged solves of the caller's own workload for a list of candidates
 (first the brick heights at a fixed group, then the groups at the best height) and records
@@ -61,9 +61,22 @@ def load(path):
 
 def default3d(nsrc):
     """The library's defaults for nsrc sources (fteik3d_solve with tile_a = group = 0):
-    brick height 0 = 24 planes from 32 sources, 16 from 16, else 8; 2 sources per
-    workgroup from 8 sources."""
+    brick height 0 (``default_tile``), 2 sources per workgroup from 8 sources."""
     return 0, (2 if nsrc >= 8 else 1)
+
+
+def default_tile(nsrc, nz_nodes):
+    """The brick height the library uses for tile_a = 0 (sweep3d.hip default_tile): from
+    32 sources the height t in 20..28 with the fewest steps down a column of nz_nodes node
+    planes, ceil(nz_nodes / t) * (t + 14) (the taller on ties); 16 from 16 sources, else 8."""
+    if nsrc < 32:
+        return 16 if nsrc >= 16 else 8
+    best, best_steps = 24, None
+    for t in range(20, 29):
+        steps = -(-nz_nodes // t) * (t + 14)
+        if best_steps is None or steps <= best_steps:
+            best, best_steps = t, steps
+    return best
 
 
 def tune3d(eik, sources, *, tiles=TILES, groups=GROUPS, max_sets=100, use_graph=True,

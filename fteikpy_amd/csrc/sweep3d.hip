@@ -988,10 +988,24 @@ int launch_seed3d(const SweepArgs &a, const Ws3 &w, const double *slow, int nsrc
     return hip_status(hipGetLastError());
 }
 
-// brick height default: 24-plane bricks for batches (shorter bricks are skipped more often
-// in the sparse later sets, taller ones waste less on the 14-step skew), 16 from 16
-// sources, 8 planes for a few sources (latency bound).  Measured on MI355X.
-int default_tile(int nsrc) { return nsrc >= 32 ? 24 : (nsrc >= 16 ? 16 : 8); }
+// brick height default for NZ node planes.  Batches (>= 32 sources): between 20 and 28
+// planes (shorter bricks are skipped more often in the sparse later sets, taller ones waste
+// less on the 14-step skew), the height with the fewest steps down a column -- bricks x
+// (height + 14) -- so that no short last brick pays a whole skew (the taller on ties);
+// 16 planes from 16 sources, 8 for a few sources (latency bound).  Measured on MI355X.
+int default_tile(int nsrc, int NZ) {
+    if (nsrc < 32) return nsrc >= 16 ? 16 : 8;
+    int best = 24;
+    long long best_steps = -1;
+    for (int t = 20; t <= 28; ++t) {
+        const long long steps = (long long)ceil_div(NZ, t) * (t + 14);
+        if (best_steps < 0 || steps <= best_steps) {
+            best = t;
+            best_steps = steps;
+        }
+    }
+    return best;
+}
 
 // Sweep arguments of one model and its workspace carve-up (the history fields are
 // cleared; the logged solve sets them).
@@ -1021,7 +1035,7 @@ void make_args(const double *slow, int nz, int nx, int ny, double dz, double dx,
     a.node_stride = nn;
     a.nz = nz; a.nx = nx; a.ny = ny;
     a.NZ = NZ; a.NX = NX; a.NY = NY;
-    if (tile_a <= 0) tile_a = default_tile(nsrc);
+    if (tile_a <= 0) tile_a = default_tile(nsrc, NZ);
     a.tile_a = std::min(tile_a, NZ);
     a.nba = ceil_div(NZ, a.tile_a);
     a.nbb = ceil_div(NX, kLanesB);
@@ -1102,7 +1116,7 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
     // brick height: the caller's, else the library default for this many sources
     Ws3 w;
     SweepArgs a;
-    make_args(slow, nz, nx, ny, dz, dx, dy, nsrc, tile_a > 0 ? tile_a : default_tile(nsrc), group,
+    make_args(slow, nz, nx, ny, dz, dx, dy, nsrc, tile_a > 0 ? tile_a : default_tile(nsrc, NZ), group,
               want_grad ? sgn : nullptr, tt, sets_done, (char *)workspace, &a, &w);
     a.log_node = hv ? hv->node : nullptr;
     a.log_old = hv ? hv->old : nullptr;
@@ -1323,7 +1337,8 @@ void batch3_layout(const fteik_problem *p, int n, Batch3Layout *L) {
             idx.push_back(i);
             ++n3;
             const int NZ = p[i].nz + 1;
-            const int tile = std::min(p[i].tile_a > 0 ? p[i].tile_a : default_tile(p[i].nsrc), NZ);
+            const int tile = std::min(
+                p[i].tile_a > 0 ? p[i].tile_a : default_tile(p[i].nsrc, NZ), NZ);
             dmax = std::max(dmax, ceil_div(NZ, tile) + ceil_div(p[i].nx + 1, kLanesB) +
                                       ceil_div(p[i].ny + 1, kLanesC) - 2);
         }

@@ -56,7 +56,8 @@ size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc);
  * nsweep    > 0: exactly nsweep sweep sets (the reference's nsweep);
  *           <= 0: sets until one changes no node, at most max_sets
  * want_grad record the upwind direction of every update (needed by fteik3d_gradient)
- * tile_a    nodes per brick along z (0 = library default for the shape)
+ * tile_a    nodes per brick along z (0 = library default: from 32 sources 20..28 planes,
+ *           the fewest steps down a column; 16 from 16 sources; else 8)
  * group     sources per workgroup (0 = default; 1, 2, 4, 8)
  * schedule  1: one launch per (pass, brick anti-diagonal), ordered by the stream;
  *           0: persistent (one launch per sweep set, tasks wait on per-tile progress

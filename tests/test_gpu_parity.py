@@ -230,10 +230,10 @@ def test_many_sets_skip_bitwise(fa):
 
 
 def test_default_brick_heights_bitwise(fa):
-    """The library's default brick heights (24 planes from 32 sources, 16 from 16) on a grid
-    whose 46 node planes leave a partial last brick: every intermediate nsweep and the
-    converged grids stay bit-identical to the oracle."""
-    shape, d = (45, 27, 30), (9.0, 8.0, 10.0)
+    """The library's default brick heights (27 planes for these 53 node planes from 32
+    sources, leaving a partial last brick; 16 from 16 sources): every intermediate nsweep
+    and the converged grids stay bit-identical to the oracle."""
+    shape, d = (52, 27, 30), (9.0, 8.0, 10.0)
     v = _rough_model(shape, 17)
     rng = np.random.default_rng(17)
     eik = fa.Eikonal3D(v, d)
