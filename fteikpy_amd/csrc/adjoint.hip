@@ -581,7 +581,7 @@ extern "C" int fteik3d_adjoint_history(const double *slow, int nz, int nx, int n
                                        size_t workspace_bytes, void *stream_) {
     if (!slow || !origin || !src || !tt || !history || !dlds || !workspace ||
         (nrec > 0 && (!rec || !dldt)) || nz < 1 || nx < 1 || ny < 1 || nsrc < 1 || nrec < 0 ||
-        !(dz > 0.0) || !(dx > 0.0) || !(dy > 0.0))
+        !spacing_ok3(dz, dx, dy))
         return FTEIK_ERR_ARG;
     hipStream_t stream = (hipStream_t)stream_;
     HistHeader hdr;
@@ -700,8 +700,7 @@ extern "C" int fteik3d_adjoint(const double *slow, int nz, int nx, int ny, doubl
                                double *dlds, int *stale_out, int *rounds_out, void *workspace,
                                size_t workspace_bytes, void *stream_) {
     if (!slow || !origin || !src || !tt || !dlds || !workspace || (nrec > 0 && (!rec || !dldt)) ||
-        nz < 1 || nx < 1 || ny < 1 || nsrc < 1 || nrec < 0 || !(dz > 0.0) || !(dx > 0.0) ||
-        !(dy > 0.0))
+        nz < 1 || nx < 1 || ny < 1 || nsrc < 1 || nrec < 0 || !spacing_ok3(dz, dx, dy))
         return FTEIK_ERR_ARG;
     const long long nodes = (long long)(nz + 1) * (nx + 1) * (ny + 1);
     const long long total = nodes * nsrc;

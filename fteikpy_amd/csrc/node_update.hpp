@@ -12,6 +12,14 @@ struct Stencil3 {
     double rzx, rzy, rxy, rsum;  // 1 / szx, 1 / szy, 1 / sxy, 1 / dsum, correctly rounded
 };
 
+// Grid spacings the 3D operator accepts: the division by its fixed denominators (see
+// div_fixed) needs their inverse squares, sums and reciprocals to be normal doubles, which
+// holds with a wide margin for spacings in [1e-100, 1e100]; calls outside are refused.
+inline bool spacing_ok3(double dz, double dx, double dy) {
+    auto ok = [](double h) { return h >= 1e-100 && h <= 1e100; };
+    return ok(dz) && ok(dx) && ok(dy);
+}
+
 inline Stencil3 make_stencil3(double dz, double dx, double dy) {
     Stencil3 c;
     c.dz = dz; c.dx = dx; c.dy = dy;

@@ -1091,7 +1091,7 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
         !vzero || !sets_done || !workspace)
         return FTEIK_ERR_ARG;
     if (want_grad && !sgn) return FTEIK_ERR_ARG;
-    if (!(dz > 0.0) || !(dx > 0.0) || !(dy > 0.0)) return FTEIK_ERR_ARG;
+    if (!spacing_ok3(dz, dx, dy)) return FTEIK_ERR_ARG;
     if ((long long)nx * ny > INT_MAX) return FTEIK_ERR_ARG;  // cell-plane offsets are 32-bit
     if (workspace_bytes < fteik3d_workspace_size(nz, nx, ny, nsrc)) return FTEIK_ERR_WORKSPACE;
     if (nsweep <= 0 && max_sets < 1) return FTEIK_ERR_ARG;
@@ -1397,7 +1397,7 @@ int batch3_prepare(const fteik_problem *P, int n, int nsweep, int max_sets, int 
         const fteik_problem &q = P[i];
         if (q.ndim != 3) continue;
         if (q.nz < 1 || q.nx < 1 || q.ny < 1 || q.nsrc < 1 || !q.slow || !q.src || !q.tt ||
-            !q.vzero || !q.sets_done || !(q.dz > 0.0) || !(q.dx > 0.0) || !(q.dy > 0.0) ||
+            !q.vzero || !q.sets_done || !spacing_ok3(q.dz, q.dx, q.dy) ||
             (want_grad && !q.sgn) || (long long)q.nx * q.ny > INT_MAX) {
             delete b;
             return FTEIK_ERR_ARG;

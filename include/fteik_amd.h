@@ -52,6 +52,9 @@ size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc);
  *
  * slow      device [nz, nx, ny] slowness (1/velocity), shared by all sources; one cell
  *           plane (nx * ny) must stay below 2^31 cells (else FTEIK_ERR_ARG)
+ * dz,dx,dy  grid spacings, each in [1e-100, 1e100] (else FTEIK_ERR_ARG): the operator
+ *           divides by fixed sums of inverse squared spacings through their reciprocals,
+ *           which needs them to be normal doubles
  * src       host   [nsrc, 3] source coordinates relative to the grid origin
  * nsweep    > 0: exactly nsweep sweep sets (the reference's nsweep);
  *           <= 0: sets until one changes no node, at most max_sets

@@ -242,3 +242,18 @@ def test_cell_plane_limit():
     st = L.fteik3d_solve(fake, 1, 50000, 50000, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, 0, 1, fake,
                          None, fake, fake, fake, 1 << 62, 0, None)
     assert st == _lib.ERR_ARG
+
+
+def test_spacing_range():
+    """The 3D operator's fixed-denominator division needs normal inverse squares: spacings
+    outside [1e-100, 1e100] are refused from the host (fake pointers, no device work)."""
+    from fteikpy_amd import _lib
+
+    L = _lib.lib()
+    fake = ctypes.c_void_p(0x1000)
+    src = np.array([0.5e-120, 0.5, 0.5])
+    sp = src.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    for d in ((1e-120, 1.0, 1.0), (1.0, 1e120, 1.0), (1.0, 1.0, 0.0)):
+        st = L.fteik3d_solve(fake, 4, 4, 4, *d, sp, 1, 2, 10, 0, 0, 0, 1, fake, None, fake, fake,
+                             fake, 1 << 40, 0, None)
+        assert st == _lib.ERR_ARG, d
