@@ -75,6 +75,22 @@ __device__ __forceinline__ double sqrt_rn(double x) {
     return g;
 }
 
+// sqrt_rn for code run by a single lane of its wave: the same values, with a plain branch
+// (no wave vote, so loops around it stay unrollable).
+__device__ __forceinline__ double sqrt_rn_lane(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    if (__builtin_expect(!(x >= 0x1p-767 && x < __builtin_inf()), 0)) g = sqrt(x);
+    return g;
+}
+
 // Converts a HIP error into the library's negative return code.
 inline int hip_status(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 

@@ -23,6 +23,7 @@ struct Grid2 {
     uint8_t *sgn;        // [NZ][NX] or null
     int nz, nx, NZ, NX;
     double dz, dx, dzi, dxi, dz2i, dx2i;
+    double ssum, rsum;  // dz2i + dx2i (the 4-point operator's divisor), 1 / ssum rounded
     __device__ __forceinline__ double &T(int i, int j) const { return tt[(long long)i * NX + j]; }
     __device__ __forceinline__ double S(int i, int j) const { return slow[(long long)i * nx + j]; }
     __device__ __forceinline__ void setsgn(int i, int j, int sz, int sx) const {
@@ -69,31 +70,36 @@ __device__ __forceinline__ double delta(double t1, double tauv, double taue, dou
     return dpoly >= 0.0 ? 0.5 * (sqrt(dpoly) - bpoly) / apoly + t0c : t1;
 }
 
-// 2D node update, _fteik2d.py:70-187.  Returns true if the node changed.
-__device__ inline bool node2(const Grid2 &G, const SrcParam &P, double vzero, int i, int j,
-                             int sgnvz, int sgnvx, int sgntz, int sgntx) {
-    const int NZ = G.NZ, NX = G.NX;
+// Arithmetic of the 2D node update, _fteik2d.py:70-187, on values: tv / te / tev the
+// upwind neighbours along z, along x and on the diagonal, t0 the node's current time; sza,
+// szb the cells of the z 1D operator (the reference's slow[i1, max(j-1, 0)] and
+// slow[i1, min(j, nx-1)]), sxa, sxb those of the x 1D operator (slow[max(i-1, 0), j1] and
+// slow[min(i, nz-1), j1]), vref the 2D operator's cell slow[i1, j1]; far: the node is more
+// than 5 nodes from the source cell along z or x (else the spherical operator applies).
+// Returns the new time; *sg receives which upwind signs to record: 0 unchanged, 1 z only,
+// 2 x only, 3 both.
+__device__ __forceinline__ double node2_core(const Grid2 &G, const SrcParam &P, double vzero,
+                                             int i, int j, bool far, int sgntz, int sgntx,
+                                             double tv,
+                                             double te, double tev, double t0, double sza,
+                                             double szb, double sxa, double sxb, double vref,
+                                             int *sg) {
     const double dz = G.dz, dx = G.dx;
-    const int i1 = i - sgnvz, j1 = j - sgnvx;
-    const double tv = G.T(i - sgntz, j);
-    const double te = G.T(i, j - sgntx);
-    const double tev = G.T(i - sgntz, j - sgntx);
-
-    double vref = pymin(G.S(i1, max(j - 1, 0)), G.S(i1, min(j, NX - 2)));
-    const double t1d1 = tv + dz * vref;
-    vref = pymin(G.S(max(i - 1, 0), j1), G.S(min(i, NZ - 2), j1));
-    const double t1d2 = te + dx * vref;
+    const double t1d1 = tv + dz * pymin(sza, szb);
+    const double t1d2 = te + dx * pymin(sxa, sxb);
     const double t1d = pymin(t1d1, t1d2);
 
     double t2d = kBig;
-    vref = G.S(i1, j1);
-    if (abs(i - P.zsi) > kEpsIn || abs(j - P.xsi) > kEpsIn) {
-        // plane-wave: 4-point operator, else one of two 3-point operators
+    if (far) {
+        // plane-wave: 4-point operator, else one of two 3-point operators.  The 4-point
+        // quotient and root are the correctly rounded ones (div_fixed, sqrt_rn_lane), as the
+        // reference's / and ** 0.5.
         if (tv <= te + dx * vref && te <= tv + dz * vref && te >= tev && tv >= tev) {
             const double ta = tev + te - tv, tb = tev - te + tv, q = ta - tb;
-            t2d = ((tb * G.dz2i + ta * G.dx2i) +
-                   sqrt(4.0 * (vref * vref) * (G.dz2i + G.dx2i) - G.dz2i * G.dx2i * (q * q))) /
-                  (G.dz2i + G.dx2i);
+            t2d = div_fixed((tb * G.dz2i + ta * G.dx2i) +
+                                sqrt_rn_lane(4.0 * (vref * vref) * G.ssum -
+                                             G.dz2i * G.dx2i * (q * q)),
+                            G.ssum, G.rsum);
         } else if (te - tev <= dz * dz * vref / sqrt(dx * dx + dz * dz) && te - tev > 0.0) {
             const double r = (te - tev) / dz;
             t2d = te + dx * sqrt(vref * vref - r * r);
@@ -114,16 +120,168 @@ __device__ inline bool node2(const Grid2 &G, const SrcParam &P, double vzero, in
             if (t2d < tv || t2d < te) t2d = kBig;
         }
     }
-    const double t0 = G.T(i, j);
     const double tnew = pymin(pymin(t0, t1d), t2d);
+    *sg = tnew == t0 ? 0 : (tnew == t1d1 ? 1 : (tnew == t1d2 ? 2 : 3));
+    return tnew;
+}
+
+// Upwind signs of a changed node, _fteik2d.py:176-187.
+__device__ __forceinline__ void record2(const Grid2 &G, int i, int j, int sg, int sgntz,
+                                        int sgntx) {
+    if (sg == 1) G.setsgn(i, j, sgntz, 0);
+    else if (sg == 2) G.setsgn(i, j, 0, sgntx);
+    else if (sg == 3) G.setsgn(i, j, sgntz, sgntx);
+}
+
+// 2D node update on the grid in global memory.  Returns true if the node changed.
+__device__ inline bool node2(const Grid2 &G, const SrcParam &P, double vzero, int i, int j,
+                             int sgnvz, int sgnvx, int sgntz, int sgntx) {
+    const int i1 = i - sgnvz, j1 = j - sgnvx;
+    int sg;
+    const bool far = abs(i - P.zsi) > kEpsIn || abs(j - P.xsi) > kEpsIn;
+    const double tnew = node2_core(
+        G, P, vzero, i, j, far, sgntz, sgntx, G.T(i - sgntz, j), G.T(i, j - sgntx),
+        G.T(i - sgntz, j - sgntx), G.T(i, j), G.S(i1, max(j - 1, 0)), G.S(i1, min(j, G.NX - 2)),
+        G.S(max(i - 1, 0), j1), G.S(min(i, G.NZ - 2), j1), G.S(i1, j1), &sg);
     G.T(i, j) = tnew;
-    if (tnew != t0) {
-        if (tnew == t1d1) G.setsgn(i, j, sgntz, 0);
-        else if (tnew == t1d2) G.setsgn(i, j, 0, sgntx);
-        else G.setsgn(i, j, sgntz, sgntx);
-        return true;
+    record2(G, i, j, sg, sgntz, sgntx);
+    return sg != 0;
+}
+
+// Staged layout of the 2D sweep in LDS: rows r = 0 .. NZ of 4 doubles.  Slots 0 and 1 hold
+// node r-1 of two node columns (the previous and the current one, their roles swapping
+// from column to column); slots 2 and 3 hold cell clamp(r-1, 0, nz-1) of the cell columns
+// max(j-1, 0) and min(j, nx-1).  Every operand of node i then sits in rows i .. i+2 at a
+// fixed slot: one address register and immediate offsets, no index clamping.
+constexpr int kRow2 = 4;
+
+// Largest dynamic LDS the staged sweep asks for (64 KiB, the per-workgroup allocation every
+// launch can make): models up to nz = 2046 cells deep run staged.
+constexpr long long kLds2Max = 65536;
+
+// LDS bytes of the staged sweep for an nz-cell-deep model.
+__host__ __device__ __forceinline__ long long lds2_bytes(int nz) {
+    return (long long)(nz + 2) * kRow2 * (long long)sizeof(double);
+}
+
+// The +z and -z passes over node column j of one x direction (SX = sgntx), the previous
+// column (final for this half set) in slot PV, column j in slot 1 - PV; _fteik2d.py:199-287.
+// Run by one lane: the upwind z neighbour is the previous update's result, kept in a
+// register, and the next node's operands are read before the current update.  Returns
+// nonzero if a node changed.
+template <int SX, int PV, bool GRAD>
+__device__ __forceinline__ int column2(const Grid2 &G, const SrcParam &P, double vzero, int j,
+                                       double *lds) {
+    constexpr int CU = 1 - PV;
+    constexpr int CJ = SX > 0 ? 2 : 3;  // cells of column j1 = j - sgnvx
+    const int NZ = G.NZ;
+    int ch = 0, sg;
+    // node i is far from the source (node2_core) unless (unsigned)(i - nlo) <= 2 kEpsIn
+    const int nlo = abs(j - P.xsi) > kEpsIn ? (1 << 30) : P.zsi - kEpsIn;
+    // +z: i = 1 .. NZ-1 (i1 = i - 1); node i's operands in rows i, i+1
+    const double *b = lds + kRow2;
+    double tv = b[CU];
+    double tev = b[PV], te = b[kRow2 + PV], t0 = b[kRow2 + CU];
+    double sza = b[2], szb = b[3], sxa = b[CJ], sxb = b[kRow2 + CJ];
+#pragma unroll 2
+    for (int i = 1; i < NZ; ++i) {
+        const double *bn = lds + kRow2 * (i + 1 < NZ ? i + 1 : i);  // the last reads itself
+        const double tev_n = bn[PV], te_n = bn[kRow2 + PV], t0_n = bn[kRow2 + CU];
+        const double sza_n = bn[2], szb_n = bn[3], sxa_n = bn[CJ], sxb_n = bn[kRow2 + CJ];
+        const bool far = (unsigned)(i - nlo) > 2u * kEpsIn;
+        const double tnew = node2_core(G, P, vzero, i, j, far, 1, SX, tv, te, tev, t0, sza, szb,
+                                       sxa, sxb, sxa, &sg);
+        lds[kRow2 * (i + 1) + CU] = tnew;
+        if (GRAD) record2(G, i, j, sg, 1, SX);
+        ch |= sg;
+        tv = tnew;
+        te = te_n; tev = tev_n; t0 = t0_n;
+        sza = sza_n; szb = szb_n; sxa = sxa_n; sxb = sxb_n;
     }
-    return false;
+    // -z: i = NZ-2 .. 0 (i1 = i); node i's operands in rows i .. i+2; tv continues from the
+    // +z pass's result at NZ-1
+    b = lds + kRow2 * (NZ - 2);
+    te = b[kRow2 + PV]; tev = b[2 * kRow2 + PV]; t0 = b[kRow2 + CU];
+    sza = b[kRow2 + 2]; szb = b[kRow2 + 3]; sxa = b[CJ]; sxb = b[kRow2 + CJ];
+#pragma unroll 2
+    for (int i = NZ - 2; i >= 0; --i) {
+        const double *bn = lds + kRow2 * (i > 0 ? i - 1 : i);
+        const double te_n = bn[kRow2 + PV], tev_n = bn[2 * kRow2 + PV], t0_n = bn[kRow2 + CU];
+        const double sza_n = bn[kRow2 + 2], szb_n = bn[kRow2 + 3], sxa_n = bn[CJ];
+        const double sxb_n = bn[kRow2 + CJ];
+        const bool far = (unsigned)(i - nlo) > 2u * kEpsIn;
+        const double tnew = node2_core(G, P, vzero, i, j, far, -1, SX, tv, te, tev, t0, sza, szb,
+                                       sxa, sxb, sxb, &sg);
+        lds[kRow2 * (i + 1) + CU] = tnew;
+        if (GRAD) record2(G, i, j, sg, -1, SX);
+        ch |= sg;
+        tv = tnew;
+        te = te_n; tev = tev_n; t0 = t0_n;
+        sza = sza_n; szb = szb_n; sxa = sxa_n; sxb = sxb_n;
+    }
+    return ch;
+}
+
+// Node column j into slot CU, with the cells of columns max(j-1, 0) and min(j, nx-1); all
+// lanes (lane l always moves rows l, l+64, ...).
+__device__ __forceinline__ void stage2(const Grid2 &G, int j, int cu, double *lds) {
+    const int jl = max(j - 1, 0), jr = min(j, G.nx - 1);
+#pragma unroll 4
+    for (int r = threadIdx.x; r <= G.NZ; r += 64) {
+        const int k = min(max(r - 1, 0), G.nz - 1);
+        if (r > 0) lds[kRow2 * r + cu] = G.T(r - 1, j);
+        lds[kRow2 * r + 2] = G.S(k, jl);
+        lds[kRow2 * r + 3] = G.S(k, jr);
+    }
+}
+
+// Slot cu of the staged rows back to node column j; all lanes, the rows of stage2.
+__device__ __forceinline__ void unstage2(const Grid2 &G, int j, int cu, const double *lds) {
+    for (int r = threadIdx.x; r <= G.NZ; r += 64)
+        if (r > 0) G.T(r - 1, j) = lds[kRow2 * r + cu];
+}
+
+template <int SX, bool GRAD>
+__device__ __forceinline__ int column2_pv(const Grid2 &G, const SrcParam &P, double vzero, int j,
+                                          int pv, double *lds) {
+    return pv ? column2<SX, 1, GRAD>(G, P, vzero, j, lds) : column2<SX, 0, GRAD>(G, P, vzero, j, lds);
+}
+
+// Sweep sets of one source with the two active node columns and their cells staged in
+// LDS (lds2_bytes(nz)), in the reference's order, _fteik2d.py:190-287.  Lane 0 runs the
+// serial chain; the wave's 64 lanes move columns between global memory and LDS.  Returns
+// the number of sets run.
+template <bool GRAD>
+__device__ __forceinline__ int sets2_lds(const Grid2 &G, const SrcParam &P, double vzero,
+                                         int cap, double *lds) {
+    const int NX = G.NX;
+    int pv = 0;  // slot of the previous column
+    for (int r = threadIdx.x; r <= G.NZ; r += 64)
+        if (r > 0) lds[kRow2 * r + pv] = G.T(r - 1, 0);
+    int done = 0;
+    for (int set = 0; set < cap; ++set) {
+        int ch = 0;
+        for (int j = 1; j < NX; ++j) {  // +x: previous = column j-1
+            stage2(G, j, 1 - pv, lds);
+            __syncthreads();
+            if (threadIdx.x == 0) ch |= column2_pv<1, GRAD>(G, P, vzero, j, pv, lds);
+            __syncthreads();
+            unstage2(G, j, 1 - pv, lds);
+            pv = 1 - pv;
+        }
+        for (int j = NX - 2; j >= 0; --j) {  // -x: previous = column j+1
+            stage2(G, j, 1 - pv, lds);
+            __syncthreads();
+            if (threadIdx.x == 0) ch |= column2_pv<-1, GRAD>(G, P, vzero, j, pv, lds);
+            __syncthreads();
+            unstage2(G, j, 1 - pv, lds);
+            pv = 1 - pv;
+        }
+        ++done;
+        // a set that changed nothing leaves every later set a no-op
+        if (!__syncthreads_or(ch)) break;
+    }
+    return done;
 }
 
 __global__ void fill2d_kernel(double *tt, long long n, uint8_t *sgn) {
@@ -134,11 +292,13 @@ __global__ void fill2d_kernel(double *tt, long long n, uint8_t *sgn) {
     }
 }
 
-// Source initialisation (_fteik2d.py:355-617) and the sweep sets of source s, by one lane.
+// Source initialisation (_fteik2d.py:355-617) and the sweep sets of source s.  lds null:
+// lane 0 does everything on the grid in global memory; else lds holds lds2_bytes(nz) and
+// the sets run staged (sets2_lds) by the whole wave.
 __device__ __forceinline__ void solve_source2(const double *slow, int nz, int nx, double dz, double dx,
                               const SrcParam &P, int s, int nsweep, int max_sets,
                               double *tt_all, uint8_t *sgn_all, double *vzero_out,
-                              int *sets_done) {
+                              int *sets_done, double *lds) {
     Grid2 G;
     G.slow = slow;
     G.nz = nz;
@@ -151,11 +311,12 @@ __device__ __forceinline__ void solve_source2(const double *slow, int nz, int nx
     G.dz = dz;
     G.dx = dx;
     const double vzero = slow[(long long)P.zsi * nx + P.xsi];
-    vzero_out[s] = vzero;
     const int zsi = P.zsi, xsi = P.xsi, NZ = G.NZ, NX = G.NX;
     const double zsa = P.zsa, xsa = P.xsa;
 
-    if (P.iflag == 2) {
+    if (threadIdx.x != 0) {
+        // source initialisation: lane 0 only
+    } else if (P.iflag == 2) {
         const double dzu = fabs(zsa - (double)zsi), dzd = 1.0 - dzu;
         const double dxw = fabs(xsa - (double)xsi), dxe = 1.0 - dxw;
         for (int c = 0; c < 4; ++c) {  // corners (0,0) (1,0) (0,1) (1,1)
@@ -276,7 +437,21 @@ __device__ __forceinline__ void solve_source2(const double *slow, int nz, int nx
     G.dxi = 1.0 / dx;
     G.dz2i = G.dzi / dz;
     G.dx2i = G.dxi / dx;
+    G.ssum = G.dz2i + G.dx2i;
+    G.rsum = 1.0 / G.ssum;
     const int cap = nsweep > 0 ? nsweep : max_sets;
+    if (lds) {
+        __syncthreads();  // lane 0's initialisation before the wave reads column 0
+        const int done = G.sgn ? sets2_lds<true>(G, P, vzero, cap, lds)
+                               : sets2_lds<false>(G, P, vzero, cap, lds);
+        if (threadIdx.x == 0) {
+            vzero_out[s] = vzero;
+            sets_done[s] = done;
+        }
+        return;
+    }
+    if (threadIdx.x != 0) return;
+    vzero_out[s] = vzero;
     int done = 0;
     for (int set = 0; set < cap; ++set) {
         bool ch = false;
@@ -295,20 +470,21 @@ __device__ __forceinline__ void solve_source2(const double *slow, int nz, int nx
     sets_done[s] = done;
 }
 
-// One lane per block (see the launches): the bound lets the compiler keep the whole
-// per-source chain in registers.
+// One 64-lane wave per source (block s); staged: the model fits the launch's dynamic LDS
+// (lds2_bytes).  The bound lets the compiler keep the whole per-source chain in registers.
 __global__ __launch_bounds__(64) void fteik2d_kernel(const double *slow, int nz, int nx, double dz,
                                                      double dx,
                                const SrcParam *prm, int nsrc, int nsweep, int max_sets,
                                double *tt_all, uint8_t *sgn_all, double *vzero_out,
-                               int *sets_done) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+                               int *sets_done, int staged) {
+    extern __shared__ double lds2[];
+    const int s = blockIdx.x;
     if (s >= nsrc) return;
     solve_source2(slow, nz, nx, dz, dx, prm[s], s, nsweep, max_sets, tt_all, sgn_all, vzero_out,
-                  sets_done);
+                  sets_done, staged ? lds2 : nullptr);
 }
 
-// Batch of 2D models: lane t takes source t - cum[m] of the model m with
+// Batch of 2D models: block t takes source t - cum[m] of the model m with
 // cum[m] <= t < cum[m+1].
 struct Prob2 {
     const double *slow;
@@ -335,9 +511,12 @@ struct Prob2Dev {
 };
 static_assert(sizeof(Prob2Dev) == sizeof(Prob2), "same table layout");
 
+// lds_bytes: the launch's dynamic LDS; models that fit it run staged.
 __global__ __launch_bounds__(64) void fteik2d_batch_kernel(const Prob2 *probs, const int *cum,
-                                                           int nmodel, int nsweep, int max_sets) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+                                                           int nmodel, int nsweep, int max_sets,
+                                                           long long lds_bytes) {
+    extern __shared__ double lds2[];
+    const int t = blockIdx.x;
     if (t >= cum[nmodel]) return;
     int lo = 0, hi = nmodel - 1;
     while (lo < hi) {
@@ -349,7 +528,8 @@ __global__ __launch_bounds__(64) void fteik2d_batch_kernel(const Prob2 *probs, c
     const int s = t - cum[lo];
     const SrcParam P = ((const SrcParam *)Q.prm)[s];  // read once, before the chain
     solve_source2((const double *)Q.slow, Q.nz, Q.nx, Q.dz, Q.dx, P, s, nsweep, max_sets,
-                  (double *)Q.tt, (uint8_t *)Q.sgn, (double *)Q.vzero, (int *)Q.sets_done);
+                  (double *)Q.tt, (uint8_t *)Q.sgn, (double *)Q.vzero, (int *)Q.sets_done,
+                  lds2_bytes(Q.nz) <= lds_bytes ? lds2 : nullptr);
 }
 
 // 2D gradient post-pass, _fteik2d.py:623-639 (seeded analytic components survive where
@@ -412,6 +592,14 @@ bool place_source2(const double *s, int nz, int nx, double dz, double dx, SrcPar
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
+// Grid spacings the 2D operator accepts: its 4-point quotient divides by the fixed
+// dz^-2 + dx^-2 through its reciprocal (div_fixed), which needs both to be normal doubles,
+// as they are with a wide margin for spacings in [1e-100, 1e100]; calls outside are refused.
+bool spacing_ok2(double dz, double dx) {
+    auto ok = [](double h) { return h >= 1e-100 && h <= 1e100; };
+    return ok(dz) && ok(dx);
+}
+
 }  // namespace
 }  // namespace fteik
 
@@ -424,7 +612,7 @@ extern "C" int fteik2d_solve(const double *slow, int nz, int nx, double dz, doub
                              double *tt, uint8_t *sgn, double *vzero, int *sets_done,
                              void *workspace, size_t workspace_bytes, void *stream_) {
     if (nz < 1 || nx < 1 || nsrc < 1 || !slow || !src || !tt || !vzero || !sets_done ||
-        !workspace || !(dz > 0.0) || !(dx > 0.0))
+        !workspace || !spacing_ok2(dz, dx))
         return FTEIK_ERR_ARG;
     if (want_grad && !sgn) return FTEIK_ERR_ARG;
     if (nsweep <= 0 && max_sets < 1) return FTEIK_ERR_ARG;
@@ -443,11 +631,13 @@ extern "C" int fteik2d_solve(const double *slow, int nz, int nx, double dz, doub
     hipLaunchKernelGGL(fill2d_kernel, dim3(blocks), dim3(256), 0, stream, tt, total,
                        want_grad ? sgn : (uint8_t *)nullptr);
     FTEIK_CHECK(hipGetLastError());
-    // one single-lane wave per source (lanes of one wave would execute the union of the
-    // branches their nodes take)
-    hipLaunchKernelGGL(fteik2d_kernel, dim3(nsrc), dim3(1), 0, stream, slow, nz, nx,
-                       dz, dx, dprm, nsrc, nsweep, max_sets, tt, want_grad ? sgn : nullptr, vzero,
-                       sets_done);
+    // one wave per source: its lane 0 runs the serial chain (lanes of one wave would
+    // execute the union of the branches their nodes take), the others move columns
+    const long long lds = lds2_bytes(nz);
+    const int staged = lds <= kLds2Max;
+    hipLaunchKernelGGL(fteik2d_kernel, dim3(nsrc), dim3(64), staged ? (unsigned)lds : 0u, stream,
+                       slow, nz, nx, dz, dx, dprm, nsrc, nsweep, max_sets, tt,
+                       want_grad ? sgn : nullptr, vzero, sets_done, staged);
     return hip_status(hipGetLastError());
 }
 
@@ -524,7 +714,7 @@ int batch2_prepare(const fteik_problem *P, int n, int nsweep, int max_sets, int 
         const fteik_problem &q = P[i];
         if (q.ndim != 2) continue;
         if (q.nz < 1 || q.nx < 1 || q.nsrc < 1 || !q.slow || !q.src || !q.tt || !q.vzero ||
-            !q.sets_done || !(q.dz > 0.0) || !(q.dx > 0.0) || (want_grad && !q.sgn) ||
+            !q.sets_done || !spacing_ok2(q.dz, q.dx) || (want_grad && !q.sgn) ||
             (long long)b->cum.back() + q.nsrc >= (1LL << 31)) {
             delete b;
             return FTEIK_ERR_ARG;
@@ -579,13 +769,18 @@ int batch2_enqueue(Batch2 *b, hipStream_t stream) {
                            b->grad ? q.sgn : (uint8_t *)nullptr);
         FTEIK_CHECK(hipGetLastError());
     }
-    // one single-lane wave per source: sources of different models (and different branches
-    // of the node update) would otherwise serialise each other inside a wave; measured 2x
-    // faster on the uneven batch
-    const int lanes = b->cum.back();
-    hipLaunchKernelGGL(fteik2d_batch_kernel, dim3(lanes), dim3(1), 0, stream,
-                       (const Prob2 *)(b->base + b->off_probs),
-                       (const int *)(b->base + b->off_cum), n2, b->nsweep, b->max_sets);
+    // one wave per source, its lane 0 running the chain: sources of different models (and
+    // different branches of the node update) would otherwise serialise each other inside a
+    // wave; measured 2x faster on the uneven batch.  Dynamic LDS: the largest model that
+    // fits kLds2Max; larger ones run on global memory.
+    long long lds = 0;
+    for (int m = 0; m < n2; ++m) {
+        const long long l = lds2_bytes(b->probs[m].nz);
+        if (l <= kLds2Max && l > lds) lds = l;
+    }
+    hipLaunchKernelGGL(fteik2d_batch_kernel, dim3(b->cum.back()), dim3(64), (unsigned)lds,
+                       stream, (const Prob2 *)(b->base + b->off_probs),
+                       (const int *)(b->base + b->off_cum), n2, b->nsweep, b->max_sets, lds);
     return hip_status(hipGetLastError());
 }
 

@@ -183,7 +183,10 @@ size_t fteik2d_workspace_size(int nsrc);
  * called from Eikonal2D.solve (fteikpy/_solver.py:48-54).  Same conventions as
  * fteik3d_solve with src [nsrc, 2] and tt [nsrc, nz+1, nx+1].  The reference's 2D set is
  * a single serial dependency chain (columns interleave the +z and -z passes), so each
- * source is swept in the reference's exact order by one GPU lane; sources run in parallel.
+ * source is swept in the reference's exact order by one GPU lane; sources run in parallel
+ * (one wave each: models up to nz = 2046 cells deep stage the two active columns in LDS,
+ * deeper ones sweep in global memory, with the same results).  dz, dx in [1e-100, 1e100]
+ * (else FTEIK_ERR_ARG), as for fteik3d_solve.
  */
 int fteik2d_solve(const double *slow, int nz, int nx, double dz, double dx, const double *src,
                   int nsrc, int nsweep, int max_sets, int want_grad, double *tt, uint8_t *sgn,
@@ -225,8 +228,8 @@ size_t fteik_batch_workspace_size(const fteik_problem *probs, int nprob);
  * problem's outputs equal those of its own fteik3d_solve / fteik2d_solve call bit for bit.
  * The 3D models share every sweep launch -- one launch per (pass, brick anti-diagonal)
  * covers the bricks of all models and their active sources -- and converge per source;
- * the 2D models (one lane per source, the reference's serial order) run concurrently on a
- * library-owned stream joined back into `stream`.
+ * the 2D models (one wave per source whose lane 0 runs the reference's serial order) run
+ * concurrently on a library-owned stream joined back into `stream`.
  *
  * nsweep, max_sets  as fteik3d_solve (nsweep <= 0: converge, at most max_sets sets)
  * want_grad         record upwind signs: every problem must then pass `sgn`

@@ -245,8 +245,9 @@ def test_cell_plane_limit():
 
 
 def test_spacing_range():
-    """The 3D operator's fixed-denominator division needs normal inverse squares: spacings
-    outside [1e-100, 1e100] are refused from the host (fake pointers, no device work)."""
+    """The 2D and 3D operators' fixed-denominator division needs normal inverse squares:
+    spacings outside [1e-100, 1e100] are refused from the host (fake pointers, no device
+    work)."""
     from fteikpy_amd import _lib
 
     L = _lib.lib()
@@ -256,4 +257,10 @@ def test_spacing_range():
     for d in ((1e-120, 1.0, 1.0), (1.0, 1e120, 1.0), (1.0, 1.0, 0.0)):
         st = L.fteik3d_solve(fake, 4, 4, 4, *d, sp, 1, 2, 10, 0, 0, 0, 1, fake, None, fake, fake,
                              fake, 1 << 40, 0, None)
+        assert st == _lib.ERR_ARG, d
+    src2 = np.array([0.5, 0.5])
+    sp2 = src2.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    for d in ((1e-120, 1.0), (1.0, 1e120), (float("nan"), 1.0)):
+        st = L.fteik2d_solve(fake, 4, 4, *d, sp2, 1, 2, 10, 0, fake, None, fake, fake, fake,
+                             1 << 40, None)
         assert st == _lib.ERR_ARG, d
