@@ -70,35 +70,56 @@ __device__ __forceinline__ double delta(double t1, double tauv, double taue, dou
     return dpoly >= 0.0 ? 0.5 * (sqrt(dpoly) - bpoly) / apoly + t0c : t1;
 }
 
-// Arithmetic of the 2D node update, _fteik2d.py:70-187, on values: tv / te / tev the
-// upwind neighbours along z, along x and on the diagonal, t0 the node's current time; sza,
-// szb the cells of the z 1D operator (the reference's slow[i1, max(j-1, 0)] and
-// slow[i1, min(j, nx-1)]), sxa, sxb those of the x 1D operator (slow[max(i-1, 0), j1] and
-// slow[min(i, nz-1), j1]), vref the 2D operator's cell slow[i1, j1]; far: the node is more
-// than 5 nodes from the source cell along z or x (else the spherical operator applies).
-// Returns the new time; *sg receives which upwind signs to record: 0 unchanged, 1 z only,
-// 2 x only, 3 both.
+// The parts of a 2D node update that do not depend on the upwind z neighbour tv, from the
+// reference's operands (_fteik2d.py:97-132): te / tev the upwind neighbours along x and on
+// the diagonal; sza, szb the cells of the z 1D operator (slow[i1, max(j-1, 0)] and
+// slow[i1, min(j, nx-1)]); sxa, sxb those of the x 1D operator (slow[max(i-1, 0), j1] and
+// slow[min(i, nz-1), j1]); vref the 2D operator's cell slow[i1, j1].  Each value is the
+// reference's own subexpression, evaluated in its order.
+struct Node2Pre {
+    double cz;    // dz * min(sza, szb): t1d1 = tv + cz
+    double t1d2;  // te + dx * min(sxa, sxb)
+    double A;     // te + dx * vref where te >= tev, else NaN (the 4-point test then fails)
+    double B;     // dz * vref
+    double K;     // 4 vref^2 (dz^-2 + dx^-2), the 4-point root's leading term
+};
+
+__device__ __forceinline__ double nan2() { return __builtin_nan(""); }
+
+__device__ __forceinline__ Node2Pre node2_pre(const Grid2 &G, double te, double tev, double sza,
+                                              double szb, double sxa, double sxb, double vref) {
+    Node2Pre q;
+    q.cz = G.dz * pymin(sza, szb);
+    q.t1d2 = te + G.dx * pymin(sxa, sxb);
+    q.A = te >= tev ? te + G.dx * vref : nan2();
+    q.B = G.dz * vref;
+    q.K = 4.0 * (vref * vref) * G.ssum;
+    return q;
+}
+
+// 2D node update, _fteik2d.py:70-187: tv, te, tev the upwind neighbours, t0 the node's
+// current time, q = node2_pre(...), vref the 2D operator's cell; far: the node is more than
+// 5 nodes from the source cell along z or x (else the spherical operator applies).  Returns
+// the new time; *sg receives which upwind signs to record: 0 unchanged, 1 z only, 2 x only,
+// 3 both.
 __device__ __forceinline__ double node2_core(const Grid2 &G, const SrcParam &P, double vzero,
                                              int i, int j, bool far, int sgntz, int sgntx,
-                                             double tv,
-                                             double te, double tev, double t0, double sza,
-                                             double szb, double sxa, double sxb, double vref,
-                                             int *sg) {
+                                             double tv, double te, double tev, double t0,
+                                             const Node2Pre &q, double vref, int *sg) {
     const double dz = G.dz, dx = G.dx;
-    const double t1d1 = tv + dz * pymin(sza, szb);
-    const double t1d2 = te + dx * pymin(sxa, sxb);
-    const double t1d = pymin(t1d1, t1d2);
+    const double t1d1 = tv + q.cz;
+    const double t1d = pymin(t1d1, q.t1d2);
 
     double t2d = kBig;
     if (far) {
-        // plane-wave: 4-point operator, else one of two 3-point operators.  The 4-point
-        // quotient and root are the correctly rounded ones (div_fixed, sqrt_rn_lane), as the
-        // reference's / and ** 0.5.
-        if (tv <= te + dx * vref && te <= tv + dz * vref && te >= tev && tv >= tev) {
-            const double ta = tev + te - tv, tb = tev - te + tv, q = ta - tb;
+        // plane-wave: 4-point operator (tv <= te + dx vref, te <= tv + dz vref, te >= tev,
+        // tv >= tev), else one of two 3-point operators.  The 4-point quotient and root
+        // are the correctly rounded ones (div_fixed, sqrt_rn_lane), as the reference's /
+        // and ** 0.5.
+        if (tv <= q.A && te <= tv + q.B && tv >= tev) {
+            const double ta = tev + te - tv, tb = tev - te + tv, d = ta - tb;
             t2d = div_fixed((tb * G.dz2i + ta * G.dx2i) +
-                                sqrt_rn_lane(4.0 * (vref * vref) * G.ssum -
-                                             G.dz2i * G.dx2i * (q * q)),
+                                sqrt_rn_lane(q.K - G.dz2i * G.dx2i * (d * d)),
                             G.ssum, G.rsum);
         } else if (te - tev <= dz * dz * vref / sqrt(dx * dx + dz * dz) && te - tev > 0.0) {
             const double r = (te - tev) / dz;
@@ -121,7 +142,7 @@ __device__ __forceinline__ double node2_core(const Grid2 &G, const SrcParam &P, 
         }
     }
     const double tnew = pymin(pymin(t0, t1d), t2d);
-    *sg = tnew == t0 ? 0 : (tnew == t1d1 ? 1 : (tnew == t1d2 ? 2 : 3));
+    *sg = tnew == t0 ? 0 : (tnew == t1d1 ? 1 : (tnew == q.t1d2 ? 2 : 3));
     return tnew;
 }
 
@@ -137,27 +158,31 @@ __device__ __forceinline__ void record2(const Grid2 &G, int i, int j, int sg, in
 __device__ inline bool node2(const Grid2 &G, const SrcParam &P, double vzero, int i, int j,
                              int sgnvz, int sgnvx, int sgntz, int sgntx) {
     const int i1 = i - sgnvz, j1 = j - sgnvx;
-    int sg;
+    const double tv = G.T(i - sgntz, j), te = G.T(i, j - sgntx);
+    const double tev = G.T(i - sgntz, j - sgntx), vref = G.S(i1, j1);
+    const Node2Pre q = node2_pre(G, te, tev, G.S(i1, max(j - 1, 0)), G.S(i1, min(j, G.NX - 2)),
+                                 G.S(max(i - 1, 0), j1), G.S(min(i, G.NZ - 2), j1), vref);
     const bool far = abs(i - P.zsi) > kEpsIn || abs(j - P.xsi) > kEpsIn;
-    const double tnew = node2_core(
-        G, P, vzero, i, j, far, sgntz, sgntx, G.T(i - sgntz, j), G.T(i, j - sgntx),
-        G.T(i - sgntz, j - sgntx), G.T(i, j), G.S(i1, max(j - 1, 0)), G.S(i1, min(j, G.NX - 2)),
-        G.S(max(i - 1, 0), j1), G.S(min(i, G.NZ - 2), j1), G.S(i1, j1), &sg);
+    int sg;
+    const double tnew = node2_core(G, P, vzero, i, j, far, sgntz, sgntx, tv, te, tev, G.T(i, j),
+                                   q, vref, &sg);
     G.T(i, j) = tnew;
     record2(G, i, j, sg, sgntz, sgntx);
     return sg != 0;
 }
 
-// Staged layout of the 2D sweep in LDS: rows r = 0 .. NZ of 4 doubles.  Slots 0 and 1 hold
-// node r-1 of two node columns (the previous and the current one, their roles swapping
-// from column to column); slots 2 and 3 hold cell clamp(r-1, 0, nz-1) of the cell columns
-// max(j-1, 0) and min(j, nx-1).  Every operand of node i then sits in rows i .. i+2 at a
-// fixed slot: one address register and immediate offsets, no index clamping.
-constexpr int kRow2 = 4;
-
-// Largest dynamic LDS the staged sweep asks for (64 KiB, the per-workgroup allocation every
-// launch can make): models up to nz = 2046 cells deep run staged.
-constexpr long long kLds2Max = 65536;
+// Staged layout of the 2D sweep in LDS: rows r = 0 .. NZ of kRow2 doubles, row r for node
+// i = r - 1 and cell k = clamp(r - 1, 0, nz - 1) of the current column j:
+//   0, 1  node i of the previous and of the current node column (roles swap per column)
+//   2     dz * min(slow[k, max(j-1, 0)], slow[k, min(j, nx-1)])   (Node2Pre::cz)
+//   3     slow[k, j1], j1 = j - sgnvx                            (vref)
+//   4     dz * slow[k, j1]                                       (Node2Pre::B)
+//   5     4 slow[k, j1]^2 (dz^-2 + dx^-2)                        (Node2Pre::K)
+//   6     Node2Pre::t1d2 of node i (the same in both z passes)
+//   7, 8  Node2Pre::A of node i in the +z and in the -z pass
+// The +z pass reads node i's operands from rows i and i+1, the -z pass from rows i+1 and
+// i+2, each at a fixed slot: one address register with immediate offsets.
+constexpr int kRow2 = 9;
 
 // LDS bytes of the staged sweep for an nz-cell-deep model.
 __host__ __device__ __forceinline__ long long lds2_bytes(int nz) {
@@ -172,66 +197,87 @@ __host__ __device__ __forceinline__ long long lds2_bytes(int nz) {
 template <int SX, int PV, bool GRAD>
 __device__ __forceinline__ int column2(const Grid2 &G, const SrcParam &P, double vzero, int j,
                                        double *lds) {
-    constexpr int CU = 1 - PV;
-    constexpr int CJ = SX > 0 ? 2 : 3;  // cells of column j1 = j - sgnvx
+    constexpr int CU = 1 - PV, R = kRow2;
     const int NZ = G.NZ;
     int ch = 0, sg;
     // node i is far from the source (node2_core) unless (unsigned)(i - nlo) <= 2 kEpsIn
     const int nlo = abs(j - P.xsi) > kEpsIn ? (1 << 30) : P.zsi - kEpsIn;
     // +z: i = 1 .. NZ-1 (i1 = i - 1); node i's operands in rows i, i+1
-    const double *b = lds + kRow2;
+    const double *b = lds + R;
     double tv = b[CU];
-    double tev = b[PV], te = b[kRow2 + PV], t0 = b[kRow2 + CU];
-    double sza = b[2], szb = b[3], sxa = b[CJ], sxb = b[kRow2 + CJ];
+    double tev = b[PV], te = b[R + PV], t0 = b[R + CU];
+    Node2Pre q = {b[2], b[R + 6], b[R + 7], b[4], b[5]};
 #pragma unroll 2
     for (int i = 1; i < NZ; ++i) {
-        const double *bn = lds + kRow2 * (i + 1 < NZ ? i + 1 : i);  // the last reads itself
-        const double tev_n = bn[PV], te_n = bn[kRow2 + PV], t0_n = bn[kRow2 + CU];
-        const double sza_n = bn[2], szb_n = bn[3], sxa_n = bn[CJ], sxb_n = bn[kRow2 + CJ];
+        b = lds + R * i;
+        const double *bn = lds + R * (i + 1 < NZ ? i + 1 : i);  // the last reads itself
+        const double tev_n = bn[PV], te_n = bn[R + PV], t0_n = bn[R + CU];
+        const Node2Pre q_n = {bn[2], bn[R + 6], bn[R + 7], bn[4], bn[5]};
         const bool far = (unsigned)(i - nlo) > 2u * kEpsIn;
-        const double tnew = node2_core(G, P, vzero, i, j, far, 1, SX, tv, te, tev, t0, sza, szb,
-                                       sxa, sxb, sxa, &sg);
-        lds[kRow2 * (i + 1) + CU] = tnew;
+        const double tnew =
+            node2_core(G, P, vzero, i, j, far, 1, SX, tv, te, tev, t0, q, b[3], &sg);
+        lds[R * (i + 1) + CU] = tnew;
         if (GRAD) record2(G, i, j, sg, 1, SX);
         ch |= sg;
         tv = tnew;
-        te = te_n; tev = tev_n; t0 = t0_n;
-        sza = sza_n; szb = szb_n; sxa = sxa_n; sxb = sxb_n;
+        te = te_n; tev = tev_n; t0 = t0_n; q = q_n;
     }
-    // -z: i = NZ-2 .. 0 (i1 = i); node i's operands in rows i .. i+2; tv continues from the
+    // -z: i = NZ-2 .. 0 (i1 = i); node i's operands in rows i+1, i+2; tv continues from the
     // +z pass's result at NZ-1
-    b = lds + kRow2 * (NZ - 2);
-    te = b[kRow2 + PV]; tev = b[2 * kRow2 + PV]; t0 = b[kRow2 + CU];
-    sza = b[kRow2 + 2]; szb = b[kRow2 + 3]; sxa = b[CJ]; sxb = b[kRow2 + CJ];
+    b = lds + R * (NZ - 1);
+    te = b[PV]; tev = b[R + PV]; t0 = b[CU];
+    q = {b[2], b[6], b[8], b[4], b[5]};
 #pragma unroll 2
     for (int i = NZ - 2; i >= 0; --i) {
-        const double *bn = lds + kRow2 * (i > 0 ? i - 1 : i);
-        const double te_n = bn[kRow2 + PV], tev_n = bn[2 * kRow2 + PV], t0_n = bn[kRow2 + CU];
-        const double sza_n = bn[kRow2 + 2], szb_n = bn[kRow2 + 3], sxa_n = bn[CJ];
-        const double sxb_n = bn[kRow2 + CJ];
+        b = lds + R * (i + 1);
+        const double *bn = lds + R * (i > 0 ? i : i + 1);
+        const double te_n = bn[PV], tev_n = bn[R + PV], t0_n = bn[CU];
+        const Node2Pre q_n = {bn[2], bn[6], bn[8], bn[4], bn[5]};
         const bool far = (unsigned)(i - nlo) > 2u * kEpsIn;
-        const double tnew = node2_core(G, P, vzero, i, j, far, -1, SX, tv, te, tev, t0, sza, szb,
-                                       sxa, sxb, sxb, &sg);
-        lds[kRow2 * (i + 1) + CU] = tnew;
+        const double tnew =
+            node2_core(G, P, vzero, i, j, far, -1, SX, tv, te, tev, t0, q, b[3], &sg);
+        lds[R * (i + 1) + CU] = tnew;
         if (GRAD) record2(G, i, j, sg, -1, SX);
         ch |= sg;
         tv = tnew;
-        te = te_n; tev = tev_n; t0 = t0_n;
-        sza = sza_n; szb = szb_n; sxa = sxa_n; sxb = sxb_n;
+        te = te_n; tev = tev_n; t0 = t0_n; q = q_n;
     }
     return ch;
 }
 
-// Node column j into slot CU, with the cells of columns max(j-1, 0) and min(j, nx-1); all
-// lanes (lane l always moves rows l, l+64, ...).
-__device__ __forceinline__ void stage2(const Grid2 &G, int j, int cu, double *lds) {
+// Stages node column j (slot 1 - pv) with the per-row values of the layout above, the
+// previous column already in slot pv; all lanes (lane l always moves rows l, l+64, ...).
+template <int SX>
+__device__ __forceinline__ void stage2(const Grid2 &G, int j, int pv, double *lds) {
+    constexpr int R = kRow2;
+    const int cu = 1 - pv, NZ = G.NZ;
     const int jl = max(j - 1, 0), jr = min(j, G.nx - 1);
 #pragma unroll 4
-    for (int r = threadIdx.x; r <= G.NZ; r += 64) {
+    for (int r = threadIdx.x; r <= NZ; r += 64) {
         const int k = min(max(r - 1, 0), G.nz - 1);
-        if (r > 0) lds[kRow2 * r + cu] = G.T(r - 1, j);
-        lds[kRow2 * r + 2] = G.S(k, jl);
-        lds[kRow2 * r + 3] = G.S(k, jr);
+        const double sl = G.S(k, jl), sr = G.S(k, jr), v = SX > 0 ? sl : sr;
+        double *w = lds + R * r;
+        if (r > 0) w[cu] = G.T(r - 1, j);
+        w[2] = G.dz * pymin(sl, sr);
+        w[3] = v;
+        w[4] = G.dz * v;
+        w[5] = 4.0 * (v * v) * G.ssum;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r <= NZ; r += 64) {
+        if (r == 0) continue;
+        double *w = lds + R * r;
+        const double te = w[pv];
+        // x 1D operator: cells max(i-1, 0) (row r-1) and min(i, nz-1) (row r)
+        w[6] = te + G.dx * pymin(w[3 - R], w[3]);
+        if (r >= 2) {  // +z (node i >= 1): tev = previous column at i-1, vref = cell i-1
+            const double tev = w[pv - R];
+            w[7] = te >= tev ? te + G.dx * w[3 - R] : nan2();
+        }
+        if (r < NZ) {  // -z (node i <= NZ-2): tev = previous column at i+1, vref = cell i
+            const double tev = w[pv + R];
+            w[8] = te >= tev ? te + G.dx * w[3] : nan2();
+        }
     }
 }
 
@@ -247,10 +293,10 @@ __device__ __forceinline__ int column2_pv(const Grid2 &G, const SrcParam &P, dou
     return pv ? column2<SX, 1, GRAD>(G, P, vzero, j, lds) : column2<SX, 0, GRAD>(G, P, vzero, j, lds);
 }
 
-// Sweep sets of one source with the two active node columns and their cells staged in
-// LDS (lds2_bytes(nz)), in the reference's order, _fteik2d.py:190-287.  Lane 0 runs the
-// serial chain; the wave's 64 lanes move columns between global memory and LDS.  Returns
-// the number of sets run.
+// Sweep sets of one source with the two active node columns staged in LDS
+// (lds2_bytes(nz)), in the reference's order, _fteik2d.py:190-287.  Lane 0 runs the serial
+// chain; the wave's 64 lanes move columns between global memory and LDS and precompute the
+// values that do not depend on the chain.  Returns the number of sets run.
 template <bool GRAD>
 __device__ __forceinline__ int sets2_lds(const Grid2 &G, const SrcParam &P, double vzero,
                                          int cap, double *lds) {
@@ -262,7 +308,8 @@ __device__ __forceinline__ int sets2_lds(const Grid2 &G, const SrcParam &P, doub
     for (int set = 0; set < cap; ++set) {
         int ch = 0;
         for (int j = 1; j < NX; ++j) {  // +x: previous = column j-1
-            stage2(G, j, 1 - pv, lds);
+            __syncthreads();
+            stage2<1>(G, j, pv, lds);
             __syncthreads();
             if (threadIdx.x == 0) ch |= column2_pv<1, GRAD>(G, P, vzero, j, pv, lds);
             __syncthreads();
@@ -270,7 +317,8 @@ __device__ __forceinline__ int sets2_lds(const Grid2 &G, const SrcParam &P, doub
             pv = 1 - pv;
         }
         for (int j = NX - 2; j >= 0; --j) {  // -x: previous = column j+1
-            stage2(G, j, 1 - pv, lds);
+            __syncthreads();
+            stage2<-1>(G, j, pv, lds);
             __syncthreads();
             if (threadIdx.x == 0) ch |= column2_pv<-1, GRAD>(G, P, vzero, j, pv, lds);
             __syncthreads();
@@ -592,6 +640,17 @@ bool place_source2(const double *s, int nz, int nx, double dz, double dx, SrcPar
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
+// Dynamic LDS one workgroup may use on the current device (160 KiB on MI355X), the bound
+// of the staged 2D sweep: models with lds2_bytes(nz) above it sweep in global memory.
+long long lds2_limit() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        v <= 0)
+        return 65536;
+    return v;
+}
+
 // Grid spacings the 2D operator accepts: its 4-point quotient divides by the fixed
 // dz^-2 + dx^-2 through its reciprocal (div_fixed), which needs both to be normal doubles,
 // as they are with a wide margin for spacings in [1e-100, 1e100]; calls outside are refused.
@@ -634,7 +693,7 @@ extern "C" int fteik2d_solve(const double *slow, int nz, int nx, double dz, doub
     // one wave per source: its lane 0 runs the serial chain (lanes of one wave would
     // execute the union of the branches their nodes take), the others move columns
     const long long lds = lds2_bytes(nz);
-    const int staged = lds <= kLds2Max;
+    const int staged = lds <= lds2_limit();
     hipLaunchKernelGGL(fteik2d_kernel, dim3(nsrc), dim3(64), staged ? (unsigned)lds : 0u, stream,
                        slow, nz, nx, dz, dx, dprm, nsrc, nsweep, max_sets, tt,
                        want_grad ? sgn : nullptr, vzero, sets_done, staged);
@@ -772,11 +831,12 @@ int batch2_enqueue(Batch2 *b, hipStream_t stream) {
     // one wave per source, its lane 0 running the chain: sources of different models (and
     // different branches of the node update) would otherwise serialise each other inside a
     // wave; measured 2x faster on the uneven batch.  Dynamic LDS: the largest model that
-    // fits kLds2Max; larger ones run on global memory.
+    // fits the device's per-workgroup LDS; larger ones run on global memory.
     long long lds = 0;
+    const long long cap = lds2_limit();
     for (int m = 0; m < n2; ++m) {
         const long long l = lds2_bytes(b->probs[m].nz);
-        if (l <= kLds2Max && l > lds) lds = l;
+        if (l <= cap && l > lds) lds = l;
     }
     hipLaunchKernelGGL(fteik2d_batch_kernel, dim3(b->cum.back()), dim3(64), (unsigned)lds,
                        stream, (const Prob2 *)(b->base + b->off_probs),

@@ -184,8 +184,9 @@ size_t fteik2d_workspace_size(int nsrc);
  * fteik3d_solve with src [nsrc, 2] and tt [nsrc, nz+1, nx+1].  The reference's 2D set is
  * a single serial dependency chain (columns interleave the +z and -z passes), so each
  * source is swept in the reference's exact order by one GPU lane; sources run in parallel
- * (one wave each: models up to nz = 2046 cells deep stage the two active columns in LDS,
- * deeper ones sweep in global memory, with the same results).  dz, dx in [1e-100, 1e100]
+ * (one wave each: models whose (nz+2) x 72 B fit one workgroup's LDS, nz <= 2273 on
+ * MI355X, stage the two active columns in LDS, deeper ones sweep in global memory, with the
+ * same results).  dz, dx in [1e-100, 1e100]
  * (else FTEIK_ERR_ARG), as for fteik3d_solve.
  */
 int fteik2d_solve(const double *slow, int nz, int nx, double dz, double dx, const double *src,
