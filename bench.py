@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--adj-shape", type=str, default="128,256,256")
     ap.add_argument("--no-analytic", action="store_true",
                     help="skip the error report against closed-form traveltimes")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the uneven multi-model batch (48 models, 16 2D + 32 3D)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only copy GIB GiB once on the device (torch clone) and exit: the "
                          "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -139,6 +141,9 @@ def main():
     adj = fwd_adjoint(args, torch, fteikpy_amd, synthetic, dev) if args.adj_steps > 0 else None
     ana = None if args.no_analytic else analytic_errors(torch, fteikpy_amd, dev)
     c2 = None if args.no_analytic else single_source_c2(torch, fteikpy_amd, synthetic, dev)
+    ub = None if args.no_batch else uneven_batch(torch, fteikpy_amd, synthetic, dev)
+    p2d = None if args.no_analytic else plain_2d(torch, fteikpy_amd, synthetic, dev,
+                                                 with_cpu=not args.no_cpu)
 
     nodes = nz * nx * ny
     alg_bytes = sets_total * 8 * nodes * BYTES_PER_NSU
@@ -226,6 +231,8 @@ def main():
         "fwd_adjoint": adj,
         "analytic": ana,
         "single_source_200cubed": c2,
+        "uneven_batch": ub,
+        "plain_2d_101x301": p2d,
     }
     print(json.dumps(res), flush=True)
 
@@ -272,6 +279,84 @@ def single_source_c2(torch, fa, synthetic, dev, reps=3):
             "source_grids_per_s": 1.0 / best, "achieved_GBps": alg / best / 1e9,
             "note": "wall time of one converged solve (seeding, sets, convergence checks), "
                     "brick height and group at the library defaults for one source"}
+
+
+def plain_2d(torch, fa, synthetic, dev, with_cpu=True, reps=3):
+    """The plain 2D case on its one source (101 x 301 cells of 10 m, 1500 + 0.6 z m/s with
+    four Gaussian anomalies, source at (12.3, 1013.7) m): solve with gradient signs at
+    nsweep=2 and converged, then traveltimes at 200 receivers; GPU wall time (best of
+    `reps`) beside the oracle's single-thread C restatement of the reference's numba path
+    on the same solve (the 2D chain is serial per source, so one source is latency bound on
+    either side)."""
+    v, d, src, rec = synthetic.model_2d_gaussians()
+    eik = fa.Eikonal2D(v, d)
+    out = {}
+    for label, ns in (("nsweep2", 2), ("converged", None)):
+        ts = []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            g = eik.solve(src, nsweep=ns, return_gradient=True)
+            g(rec)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        entry = {"gpu_s": min(ts[1:])}
+        if with_cpu:
+            import oracle
+
+            tc = time.perf_counter()
+            oracle.solve2d(1.0 / v, *d, src[None, :], nsweep=ns or 0, grad=True, nthreads=1)
+            entry["cpu_oracle_1_thread_s"] = time.perf_counter() - tc
+        out[label] = entry
+    return out
+
+
+def uneven_batch(torch, fa, synthetic, dev, reps=3):
+    """The uneven batch of independent models (16 2D models 64 to 1024 per side and 32 3D
+    models 48^3 to 192 x 256 x 256, 1 to 32 sources each, smooth random velocities),
+    converged in one fteik_solve_batch call; beside it the 3D and the 2D models alone and
+    the 3D models solved one call each (best of `reps` after a warm-up, wall time)."""
+    models = synthetic.uneven_batch()
+    probs = [((fa.Eikonal3D if v.ndim == 3 else fa.Eikonal2D)(v, d), s) for v, d, s in models]
+    p3 = [p for p in probs if isinstance(p[0], fa.Eikonal3D)]
+    p2 = [p for p in probs if not isinstance(p[0], fa.Eikonal3D)]
+
+    slowest = {}
+
+    def best(fn, n, key):
+        r = fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(n):
+            del r
+            t0 = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        slowest[key] = max(ts)
+        return min(ts), r
+
+    with torch.cuda.stream(torch.cuda.Stream(device=dev)):
+        t_all, res = best(lambda: fa.solve_batch_tensors(probs, nsweep=None), reps, "all")
+        t3, _ = best(lambda: fa.solve_batch_tensors(p3, nsweep=None), reps, "3d")
+        t2, _ = best(lambda: fa.solve_batch_tensors(p2, nsweep=None), reps, "2d")
+        tloop, loop = best(lambda: [e.solve_tensors(s, nsweep=None) for e, s in p3], 1, "loop")
+        off = len(p2)
+        equal = all(torch.equal(res[off + i][0], loop[i][0]) for i in range(len(p3)))
+        sets = [int(r[3].max()) for r in res]
+    del res, loop
+    torch.cuda.empty_cache()
+    log(f"uneven batch: all {t_all:.2f} s, 3D {t3:.2f} s, 2D {t2:.2f} s, "
+        f"3D one call each {tloop:.2f} s")
+    return {"models_2d": len(p2), "models_3d": len(p3),
+            "sources_2d": sum(len(s) for _, s in p2), "sources_3d": sum(len(s) for _, s in p3),
+            "seconds_all": t_all, "seconds_3d_only": t3, "seconds_2d_only": t2,
+            "seconds_3d_one_call_each": tloop, "batched_3d_over_one_call_each": tloop / t3,
+            "batched_equals_one_call_each_bitwise": equal,
+            "max_sets_2d": max(sets[:off]), "max_sets_3d": max(sets[off:]),
+            "slowest_repetition_s": slowest,
+            "note": f"converged (nsweep=None); wall time, best of {reps} repetitions after a "
+                    "warm-up (the 3D one-call-each loop: 1); the 2D models run on their own "
+                    "stream beside the 3D sweeps"}
 
 
 def stream_copy(torch, dev, gib=1, reps=10):
