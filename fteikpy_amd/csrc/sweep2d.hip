@@ -189,58 +189,136 @@ __host__ __device__ __forceinline__ long long lds2_bytes(int nz) {
     return (long long)(nz + 2) * kRow2 * (long long)sizeof(double);
 }
 
+// Operands of node i in one z pass (DZ = +1: the +z pass, rows i and i+1 of the staged
+// layout; DZ = -1: the -z pass, rows i+1 and i+2), the previous column in slot PV.
+struct Ops2 {
+    double tev, te, t0, vref;
+    Node2Pre q;
+};
+template <int DZ, int PV>
+__device__ __forceinline__ Ops2 ops2(const double *lds, int i) {
+    constexpr int R = kRow2, CU = 1 - PV;
+    Ops2 o;
+    if (DZ > 0) {
+        const double *b = lds + R * i;
+        o.tev = b[PV]; o.q.cz = b[2]; o.vref = b[3]; o.q.B = b[4]; o.q.K = b[5];
+        o.te = b[R + PV]; o.t0 = b[R + CU]; o.q.t1d2 = b[R + 6]; o.q.A = b[R + 7];
+    } else {
+        const double *b = lds + R * (i + 1);
+        o.te = b[PV]; o.t0 = b[CU]; o.q.cz = b[2]; o.vref = b[3]; o.q.B = b[4]; o.q.K = b[5];
+        o.q.t1d2 = b[6]; o.q.A = b[8]; o.tev = b[R + PV];
+    }
+    return o;
+}
+
+// cnt consecutive nodes of one z pass from node i on (DZ the direction), in order, by one
+// lane: the upwind z neighbour tv is the previous update's result, kept in a register, and
+// the next node's operands are read before the current update.  tv: in, the final time of
+// the node before i; out, that of the last node.  Returns nonzero if a node changed.
+template <int SX, int PV, bool GRAD, int DZ>
+__device__ __forceinline__ int serial2(const Grid2 &G, const SrcParam &P, double vzero, int j,
+                                       int nlo, double *lds, int i, int cnt, double &tv) {
+    constexpr int CU = 1 - PV;
+    int ch = 0, sg;
+    Ops2 o = ops2<DZ, PV>(lds, i);
+#pragma unroll 2
+    for (int k = 0; k < cnt; ++k, i += DZ) {
+        const Ops2 on = ops2<DZ, PV>(lds, k + 1 < cnt ? i + DZ : i);  // the last reads itself
+        const bool far = (unsigned)(i - nlo) > 2u * kEpsIn;
+        const double tnew = node2_core(G, P, vzero, i, j, far, DZ, SX, tv, o.te, o.tev, o.t0,
+                                       o.q, o.vref, &sg);
+        lds[kRow2 * (i + 1) + CU] = tnew;
+        if (GRAD) record2(G, i, j, sg, DZ, SX);
+        ch |= sg;
+        tv = tnew;
+        o = on;
+    }
+    return ch;
+}
+
+// Speculative evaluation of n <= 64 consecutive nodes i0, i0 + DZ, ... of one z pass, node
+// k on lane k; all lanes.  Within a pass node i's new time is x_i = F_i(x_{i-DZ}) with
+// every other operand fixed, so the pass's results are the unique solution of that
+// recurrence from the exact x before i0 (tv).  Iteration t evaluates lane k on the previous
+// iteration's output of lane k-1 (lane 0 on tv; before the first iteration the outputs are
+// the nodes' current times).  A lane's output is exact once its input is: if lanes 0..e-1
+// are exact, lane e is exact after the next iteration, and so is every following lane up
+// to the first whose output differs (bitwise) from the previous iteration's, since its
+// input then already was its predecessor's exact output.  Each iteration thus finalizes at
+// least one more node, at most 64 iterations; it stops once fewer than 4 nodes were
+// finalized in an iteration (a dense dependency chain, cheaper on one lane).  The exact
+// prefix is written back; returns its length in *done and its last time in tv, and
+// nonzero on lanes whose node changed.
+template <int SX, int PV, bool GRAD, int DZ>
+__device__ __forceinline__ int spec2(const Grid2 &G, const SrcParam &P, double vzero, int j,
+                                     int nlo, double *lds, int i0, int n, double &tv,
+                                     int *done) {
+    constexpr int CU = 1 - PV, R = kRow2;
+    const int lane = threadIdx.x;
+    const int i = i0 + DZ * (lane < n ? lane : n - 1);
+    const Ops2 o = ops2<DZ, PV>(lds, i);
+    // current time of the upwind node (+z: node i-1, row i; -z: node i+1, row i+2)
+    const double up = lds[R * (DZ > 0 ? i : i + 2) + CU];
+    double g = lane == 0 ? tv : up, yp = o.t0, y;
+    const bool far = (unsigned)(i - nlo) > 2u * kEpsIn;
+    int e = 0, sg;
+    while (true) {
+        y = node2_core(G, P, vzero, i, j, far, DZ, SX, g, o.te, o.tev, o.t0, o.q, o.vref, &sg);
+        const bool changed =
+            lane < n && __double_as_longlong(y) != __double_as_longlong(yp);
+        const unsigned long long m = __ballot(changed) & (~0ull << e);
+        const int en = m ? __ffsll((unsigned long long)m) : n;  // first changed lane + 1
+        const int adv = en - e;
+        e = en;
+        if (e >= n || adv < 4) break;
+        yp = y;
+        const double s = __shfl_up(y, 1);
+        if (lane > 0) g = s;
+    }
+    int ch = 0;
+    if (lane < e) {
+        lds[R * (i + 1) + CU] = y;
+        if (GRAD) record2(G, i, j, sg, DZ, SX);
+        ch = sg;
+    }
+    tv = __shfl(y, e - 1);
+    *done = e;
+    return ch;
+}
+
 // The +z and -z passes over node column j of one x direction (SX = sgntx), the previous
 // column (final for this half set) in slot PV, column j in slot 1 - PV; _fteik2d.py:199-287.
-// Run by one lane: the upwind z neighbour is the previous update's result, kept in a
-// register, and the next node's operands are read before the current update.  Returns
-// nonzero if a node changed.
+// All lanes: each block of up to 64 nodes is first evaluated speculatively (spec2), and
+// lane 0 sweeps what speculation left in order (serial2).  Returns nonzero on lanes that
+// changed a node.
 template <int SX, int PV, bool GRAD>
 __device__ __forceinline__ int column2(const Grid2 &G, const SrcParam &P, double vzero, int j,
                                        double *lds) {
-    constexpr int CU = 1 - PV, R = kRow2;
+    constexpr int CU = 1 - PV;
     const int NZ = G.NZ;
-    int ch = 0, sg;
     // node i is far from the source (node2_core) unless (unsigned)(i - nlo) <= 2 kEpsIn
     const int nlo = abs(j - P.xsi) > kEpsIn ? (1 << 30) : P.zsi - kEpsIn;
-    // +z: i = 1 .. NZ-1 (i1 = i - 1); node i's operands in rows i, i+1
-    const double *b = lds + R;
-    double tv = b[CU];
-    double tev = b[PV], te = b[R + PV], t0 = b[R + CU];
-    Node2Pre q = {b[2], b[R + 6], b[R + 7], b[4], b[5]};
-#pragma unroll 2
-    for (int i = 1; i < NZ; ++i) {
-        b = lds + R * i;
-        const double *bn = lds + R * (i + 1 < NZ ? i + 1 : i);  // the last reads itself
-        const double tev_n = bn[PV], te_n = bn[R + PV], t0_n = bn[R + CU];
-        const Node2Pre q_n = {bn[2], bn[R + 6], bn[R + 7], bn[4], bn[5]};
-        const bool far = (unsigned)(i - nlo) > 2u * kEpsIn;
-        const double tnew =
-            node2_core(G, P, vzero, i, j, far, 1, SX, tv, te, tev, t0, q, b[3], &sg);
-        lds[R * (i + 1) + CU] = tnew;
-        if (GRAD) record2(G, i, j, sg, 1, SX);
-        ch |= sg;
-        tv = tnew;
-        te = te_n; tev = tev_n; t0 = t0_n; q = q_n;
+    int ch = 0;
+    double tv = lds[kRow2 + CU];  // node 0
+    for (int i0 = 1; i0 < NZ;) {  // +z: nodes 1 .. NZ-1
+        const int n = min(64, NZ - i0);
+        int e;
+        ch |= spec2<SX, PV, GRAD, 1>(G, P, vzero, j, nlo, lds, i0, n, tv, &e);
+        if (e < n && threadIdx.x == 0)
+            ch |= serial2<SX, PV, GRAD, 1>(G, P, vzero, j, nlo, lds, i0 + e, n - e, tv);
+        tv = __shfl(tv, 0);
+        __syncthreads();
+        i0 += n;
     }
-    // -z: i = NZ-2 .. 0 (i1 = i); node i's operands in rows i+1, i+2; tv continues from the
-    // +z pass's result at NZ-1
-    b = lds + R * (NZ - 1);
-    te = b[PV]; tev = b[R + PV]; t0 = b[CU];
-    q = {b[2], b[6], b[8], b[4], b[5]};
-#pragma unroll 2
-    for (int i = NZ - 2; i >= 0; --i) {
-        b = lds + R * (i + 1);
-        const double *bn = lds + R * (i > 0 ? i : i + 1);
-        const double te_n = bn[PV], tev_n = bn[R + PV], t0_n = bn[CU];
-        const Node2Pre q_n = {bn[2], bn[6], bn[8], bn[4], bn[5]};
-        const bool far = (unsigned)(i - nlo) > 2u * kEpsIn;
-        const double tnew =
-            node2_core(G, P, vzero, i, j, far, -1, SX, tv, te, tev, t0, q, b[3], &sg);
-        lds[R * (i + 1) + CU] = tnew;
-        if (GRAD) record2(G, i, j, sg, -1, SX);
-        ch |= sg;
-        tv = tnew;
-        te = te_n; tev = tev_n; t0 = t0_n; q = q_n;
+    for (int i0 = NZ - 2; i0 >= 0;) {  // -z: nodes NZ-2 .. 0, tv = node NZ-1 from +z
+        const int n = min(64, i0 + 1);
+        int e;
+        ch |= spec2<SX, PV, GRAD, -1>(G, P, vzero, j, nlo, lds, i0, n, tv, &e);
+        if (e < n && threadIdx.x == 0)
+            ch |= serial2<SX, PV, GRAD, -1>(G, P, vzero, j, nlo, lds, i0 - e, n - e, tv);
+        tv = __shfl(tv, 0);
+        __syncthreads();
+        i0 -= n;
     }
     return ch;
 }
@@ -294,9 +372,10 @@ __device__ __forceinline__ int column2_pv(const Grid2 &G, const SrcParam &P, dou
 }
 
 // Sweep sets of one source with the two active node columns staged in LDS
-// (lds2_bytes(nz)), in the reference's order, _fteik2d.py:190-287.  Lane 0 runs the serial
-// chain; the wave's 64 lanes move columns between global memory and LDS and precompute the
-// values that do not depend on the chain.  Returns the number of sets run.
+// (lds2_bytes(nz)), in the reference's order, _fteik2d.py:190-287, with the results of
+// that order (column2).  The wave's 64 lanes move columns between global memory and LDS and
+// precompute the values that do not depend on the serial chain.  Returns the number of
+// sets run.
 template <bool GRAD>
 __device__ __forceinline__ int sets2_lds(const Grid2 &G, const SrcParam &P, double vzero,
                                          int cap, double *lds) {
@@ -311,7 +390,7 @@ __device__ __forceinline__ int sets2_lds(const Grid2 &G, const SrcParam &P, doub
             __syncthreads();
             stage2<1>(G, j, pv, lds);
             __syncthreads();
-            if (threadIdx.x == 0) ch |= column2_pv<1, GRAD>(G, P, vzero, j, pv, lds);
+            ch |= column2_pv<1, GRAD>(G, P, vzero, j, pv, lds);
             __syncthreads();
             unstage2(G, j, 1 - pv, lds);
             pv = 1 - pv;
@@ -320,7 +399,7 @@ __device__ __forceinline__ int sets2_lds(const Grid2 &G, const SrcParam &P, doub
             __syncthreads();
             stage2<-1>(G, j, pv, lds);
             __syncthreads();
-            if (threadIdx.x == 0) ch |= column2_pv<-1, GRAD>(G, P, vzero, j, pv, lds);
+            ch |= column2_pv<-1, GRAD>(G, P, vzero, j, pv, lds);
             __syncthreads();
             unstage2(G, j, 1 - pv, lds);
             pv = 1 - pv;
