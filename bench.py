@@ -138,12 +138,17 @@ def main():
 
     del out, tt
     torch.cuda.empty_cache()
-    adj = fwd_adjoint(args, torch, fteikpy_amd, synthetic, dev) if args.adj_steps > 0 else None
-    ana = None if args.no_analytic else analytic_errors(torch, fteikpy_amd, dev)
-    c2 = None if args.no_analytic else single_source_c2(torch, fteikpy_amd, synthetic, dev)
-    ub = None if args.no_batch else uneven_batch(torch, fteikpy_amd, synthetic, dev)
-    p2d = None if args.no_analytic else plain_2d(torch, fteikpy_amd, synthetic, dev,
-                                                 with_cpu=not args.no_cpu)
+    # every section runs on the one stream above: HIP maps a process's streams onto at most
+    # GPU_MAX_HW_QUEUES hardware queues (4 by default), and the multi-model batch overlaps
+    # its 2D and 3D parts only while its side stream has a queue of its own
+    with torch.cuda.stream(stream):
+        adj = (fwd_adjoint(args, torch, fteikpy_amd, synthetic, dev) if args.adj_steps > 0
+               else None)
+        ana = None if args.no_analytic else analytic_errors(torch, fteikpy_amd, dev)
+        c2 = None if args.no_analytic else single_source_c2(torch, fteikpy_amd, synthetic, dev)
+        ub = None if args.no_batch else uneven_batch(torch, fteikpy_amd, synthetic, dev)
+        p2d = None if args.no_analytic else plain_2d(torch, fteikpy_amd, synthetic, dev,
+                                                     with_cpu=not args.no_cpu)
 
     nodes = nz * nx * ny
     alg_bytes = sets_total * 8 * nodes * BYTES_PER_NSU
@@ -185,7 +190,7 @@ def main():
                          "path, OpenMP over sources"}
 
     torch.cuda.empty_cache()
-    copy_gbs = stream_copy(torch, dev)
+    copy_gbs = stream_copy(torch, dev, stream)
     us_launch = gpu_s / launches * 1e6
     res = {
         "metric": "converged FP64 source-grids/sec 3D, batched vs plain GPU; fwd+adjoint s/iter",
@@ -335,14 +340,14 @@ def uneven_batch(torch, fa, synthetic, dev, reps=3):
         slowest[key] = max(ts)
         return min(ts), r
 
-    with torch.cuda.stream(torch.cuda.Stream(device=dev)):
-        t_all, res = best(lambda: fa.solve_batch_tensors(probs, nsweep=None), reps, "all")
-        t3, _ = best(lambda: fa.solve_batch_tensors(p3, nsweep=None), reps, "3d")
-        t2, _ = best(lambda: fa.solve_batch_tensors(p2, nsweep=None), reps, "2d")
-        tloop, loop = best(lambda: [e.solve_tensors(s, nsweep=None) for e, s in p3], 1, "loop")
-        off = len(p2)
-        equal = all(torch.equal(res[off + i][0], loop[i][0]) for i in range(len(p3)))
-        sets = [int(r[3].max()) for r in res]
+    # on the caller's current stream
+    t_all, res = best(lambda: fa.solve_batch_tensors(probs, nsweep=None), reps, "all")
+    t3, _ = best(lambda: fa.solve_batch_tensors(p3, nsweep=None), reps, "3d")
+    t2, _ = best(lambda: fa.solve_batch_tensors(p2, nsweep=None), reps, "2d")
+    tloop, loop = best(lambda: [e.solve_tensors(s, nsweep=None) for e, s in p3], 1, "loop")
+    off = len(p2)
+    equal = all(torch.equal(res[off + i][0], loop[i][0]) for i in range(len(p3)))
+    sets = [int(r[3].max()) for r in res]
     del res, loop
     torch.cuda.empty_cache()
     log(f"uneven batch: all {t_all:.2f} s, 3D {t3:.2f} s, 2D {t2:.2f} s, "
@@ -359,13 +364,12 @@ def uneven_batch(torch, fa, synthetic, dev, reps=3):
                     "stream beside the 3D sweeps"}
 
 
-def stream_copy(torch, dev, gib=1, reps=10):
+def stream_copy(torch, dev, stream, gib=1, reps=10):
     """Device-to-device copy bandwidth (STREAM copy): bytes read + written per second over
     `reps` copies of a `gib` GiB FP64 buffer, timed with HIP events on the copy stream."""
     n = gib * (1 << 27)
     a = torch.empty(n, dtype=torch.float64, device=dev).fill_(1.0)
     b = torch.empty_like(a)
-    stream = torch.cuda.Stream(device=dev)
     with torch.cuda.stream(stream):
         for _ in range(2):
             b.copy_(a)
