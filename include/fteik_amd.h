@@ -183,10 +183,11 @@ size_t fteik2d_workspace_size(int nsrc);
  * called from Eikonal2D.solve (fteikpy/_solver.py:48-54).  Same conventions as
  * fteik3d_solve with src [nsrc, 2] and tt [nsrc, nz+1, nx+1].  The reference's 2D set is
  * a single serial dependency chain (columns interleave the +z and -z passes), so each
- * source is swept in the reference's exact order by one GPU lane; sources run in parallel
- * (one wave each: models whose (nz+2) x 72 B fit one workgroup's LDS, nz <= 2273 on
- * MI355X, stage the two active columns in LDS, deeper ones sweep in global memory, with the
- * same results).  dz, dx in [1e-100, 1e100]
+ * source gets the results of the reference's exact order; sources run in parallel, one
+ * wave each.  Models whose (nz+2) x 72 B fit one workgroup's LDS (nz <= 2273 on MI355X)
+ * stage the two active columns in LDS and evaluate each z pass in speculative blocks of 64
+ * nodes that finalize only provably exact nodes; deeper ones sweep in order in global
+ * memory, with the same results.  dz, dx in [1e-100, 1e100]
  * (else FTEIK_ERR_ARG), as for fteik3d_solve.
  */
 int fteik2d_solve(const double *slow, int nz, int nx, double dz, double dx, const double *src,
@@ -229,8 +230,10 @@ size_t fteik_batch_workspace_size(const fteik_problem *probs, int nprob);
  * problem's outputs equal those of its own fteik3d_solve / fteik2d_solve call bit for bit.
  * The 3D models share every sweep launch -- one launch per (pass, brick anti-diagonal)
  * covers the bricks of all models and their active sources -- and converge per source;
- * the 2D models (one wave per source whose lane 0 runs the reference's serial order) run
- * concurrently on a library-owned stream joined back into `stream`.
+ * the 2D models (one wave per source, the reference's results per z pass) run
+ * concurrently on a library-owned stream joined back into `stream` (they overlap the 3D
+ * sweeps only while that stream has a hardware queue of its own: HIP maps a process's
+ * streams onto GPU_MAX_HW_QUEUES queues, 4 by default).
  *
  * nsweep, max_sets  as fteik3d_solve (nsweep <= 0: converge, at most max_sets sets)
  * want_grad         record upwind signs: every problem must then pass `sgn`
