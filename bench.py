@@ -59,12 +59,19 @@ def main():
                     help="skip the error report against closed-form traveltimes")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the uneven multi-model batch (48 models, 16 2D + 32 3D)")
+    ap.add_argument("--profile-2d", action="store_true",
+                    help="only run the 2D workloads (the plain 2D case at nsweep 2, the 2D "
+                         "models of the uneven batch) and print their times, for profiling")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only copy GIB GiB once on the device (torch clone) and exit: the "
                          "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
+        return
+
+    if args.profile_2d:
+        profile_2d()
         return
 
     if args.gpus != 1:
@@ -284,6 +291,34 @@ def single_source_c2(torch, fa, synthetic, dev, reps=3):
             "source_grids_per_s": 1.0 / best, "achieved_GBps": alg / best / 1e9,
             "note": "wall time of one converged solve (seeding, sets, convergence checks), "
                     "brick height and group at the library defaults for one source"}
+
+
+def profile_2d(reps=2):
+    """The 2D workloads alone: the plain 2D case (one source, nsweep 2, gradient signs, 200
+    receivers) and the 16 2D models of the uneven batch converged, `reps` times each after
+    a warm-up, on one stream."""
+    import torch
+
+    import fteikpy_amd as fa
+    from fteikpy_amd import synthetic
+
+    dev = torch.device("cuda", 0)
+    v, d, src, rec = synthetic.model_2d_gaussians()
+    eik = fa.Eikonal2D(v, d)
+    p2 = [(fa.Eikonal2D(v2, d2), s2) for v2, d2, s2 in synthetic.uneven_batch(n3=0)]
+    out = {}
+    with torch.cuda.stream(torch.cuda.Stream(device=dev)):
+        for name, fn in (("plain_2d_nsweep2", lambda: eik.solve(src, 2, True)(rec)),
+                         ("uneven_batch_2d", lambda: fa.solve_batch_tensors(p2, nsweep=None))):
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            torch.cuda.synchronize()
+            out[name + "_s"] = (time.perf_counter() - t0) / reps
+            log(f"{name}: {out[name + '_s']:.4f} s")
+    print(json.dumps(out), flush=True)
 
 
 def plain_2d(torch, fa, synthetic, dev, with_cpu=True, reps=3):
