@@ -237,6 +237,30 @@ def test_2d_deeper_than_lds_bitwise(fa):
         assert np.array_equal(np.stack([t.grid for t in tts]), ref), ns
 
 
+def test_zero_velocity_block_bitwise(fa):
+    """Zero-velocity cells (infinite slowness, no check in the reference either): candidates
+    through them come out infinite, or NaN from the fixed-divisor division where the
+    reference's division gives inf, and lose every min either way; times equal the
+    oracle's bit for bit in 2D and 3D, at nsweep 2 and converged."""
+    v2 = _rough_model((90, 70), 31)
+    v2[30:45, 20:50] = 0.0
+    d2 = (6.0, 8.0)
+    s2 = np.array([[0.1, 0.1], [0.9, 0.8]]) * (np.array(v2.shape) * d2)
+    v3 = _rough_model((20, 24, 22), 32)
+    v3[8:12, 5:15, 6:16] = 0.0
+    d3 = (9.0, 7.0, 8.0)
+    s3 = np.array([[0.1, 0.1, 0.2], [0.9, 0.8, 0.7]]) * (np.array(v3.shape) * d3)
+    with np.errstate(divide="ignore"):
+        slow2, slow3 = 1.0 / v2, 1.0 / v3
+    for ns in (2, None):
+        ref = oracle.solve2d(slow2, *d2, s2, nsweep=ns or 0)[0]
+        got = np.stack([t.grid for t in fa.Eikonal2D(v2, d2).solve(s2, nsweep=ns)])
+        assert np.array_equal(got, ref, equal_nan=True), ("2D", ns)
+        ref = oracle.solve3d(slow3, *d3, s3, nsweep=ns or 0)[0]
+        got = np.stack([t.grid for t in fa.Eikonal3D(v3, d3).solve(s3, nsweep=ns)])
+        assert np.array_equal(got, ref, equal_nan=True), ("3D", ns)
+
+
 def test_many_sets_skip_bitwise(fa):
     """Inclusion model needing ~17 sets: late sets change <1% of the nodes, so most node
     updates are skipped by the change tracking; results stay bit-identical at every
