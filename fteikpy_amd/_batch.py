@@ -5,7 +5,8 @@ No single counterpart in the reference, which solves one model per
 (solver, sources) pairs and returns, for each, exactly what ``solver.solve(sources, ...)``
 returns -- bit for bit -- but all 3D models share every sweep launch (one launch per
 (pass, brick anti-diagonal) covers the bricks and active sources of every model) and the
-2D models run meanwhile, one lane per source.  C entry point: ``fteik_solve_batch``.
+2D models run meanwhile on a stream of their own, one wave per source.  C entry point:
+``fteik_solve_batch``.
 """
 import ctypes
 
