@@ -237,6 +237,38 @@ def test_2d_deeper_than_lds_bitwise(fa):
         assert np.array_equal(np.stack([t.grid for t in tts]), ref), ns
 
 
+@pytest.mark.parametrize("case", range(12))
+def test_2d_random_models_bitwise(fa, case):
+    """Speculative 2D passes on random models: row counts around the 64-node blocks (1, 2,
+    63, 64, 65, 127, 128, 129 cells and others), smooth to cell-scale contrast, sources on
+    edges and corners and in the interior; times, sets and (every other case) gradients
+    equal the oracle's at nsweep 1, 2, 4 and converged."""
+    rng = np.random.default_rng(500 + case)
+    nz = [1, 2, 63, 64, 65, 127, 128, 129, 200, 31, 96, 257][case]
+    nx = int(rng.integers(1, 90))
+    base = _rough_model((nz, nx), 600 + case)
+    contrast = rng.uniform(0.0, 1.0)
+    v = 1500.0 + contrast * (base - 1500.0) + rng.uniform(0.0, 2000.0)
+    d = tuple(rng.uniform(2.0, 20.0, 2))
+    ext = np.array(v.shape) * d
+    pos = rng.random((3, 2))
+    pos[0] = rng.integers(0, 2, 2)  # a corner
+    pos[1, rng.integers(0, 2)] = rng.integers(0, 2)  # an edge
+    srcs = pos * ext
+    grad = case % 2 == 0
+    eik = fa.Eikonal2D(v, d)
+    for ns in (1, 2, 4, None):
+        ref, rgrad, _, k = oracle.solve2d(1.0 / v, *d, srcs, nsweep=ns or 0, grad=grad)
+        tts = eik.solve(srcs, nsweep=ns, return_gradient=grad)
+        for s, t in enumerate(tts):
+            assert np.array_equal(t.grid, ref[s]), (case, ns, s)
+            if grad:
+                g = np.stack([x.grid for x in t.gradient], -1)
+                assert np.array_equal(g, rgrad[s]), (case, ns, s)
+            if ns is None:
+                assert t.sets_done == k[s], (case, s)
+
+
 def test_zero_velocity_block_bitwise(fa):
     """Zero-velocity cells (infinite slowness, no check in the reference either): candidates
     through them come out infinite, or NaN from the fixed-divisor division where the
