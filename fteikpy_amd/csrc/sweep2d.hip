@@ -607,7 +607,8 @@ __global__ __launch_bounds__(64) void fteik2d_kernel(const double *slow, int nz,
     extern __shared__ double lds2[];
     const int s = blockIdx.x;
     if (s >= nsrc) return;
-    solve_source2(slow, nz, nx, dz, dx, prm[s], s, nsweep, max_sets, tt_all, sgn_all, vzero_out,
+    const SrcParam P = prm[s];  // in registers: the chain's stores cannot alias it
+    solve_source2(slow, nz, nx, dz, dx, P, s, nsweep, max_sets, tt_all, sgn_all, vzero_out,
                   sets_done, staged ? lds2 : nullptr);
 }
 
