@@ -1327,6 +1327,18 @@ struct Batch3Layout {
     int dmax;
 };
 
+// Brick height of a batch's 3D model: its own tile_a, else the default for the batch's
+// total 3D source count (every launch carries the bricks of all models, so the work per
+// launch follows the total).  Measured on MI355X on the 48-model workload (32 3D models,
+// 520 sources): 0.94 s against 1.04 s with each model's own count.
+int batch_tile(const fteik_problem *p, int n, const fteik_problem &q) {
+    if (q.tile_a > 0) return q.tile_a;
+    long long nsrc3 = 0;
+    for (int i = 0; i < n; ++i)
+        if (p[i].ndim == 3 && p[i].nsrc > 0) nsrc3 += p[i].nsrc;
+    return default_tile((int)std::min<long long>(nsrc3, INT_MAX), q.nz + 1);
+}
+
 void batch3_layout(const fteik_problem *p, int n, Batch3Layout *L) {
     int n3 = 0, dmax = 0;
     L->model_off.clear();
@@ -1337,8 +1349,7 @@ void batch3_layout(const fteik_problem *p, int n, Batch3Layout *L) {
             idx.push_back(i);
             ++n3;
             const int NZ = p[i].nz + 1;
-            const int tile = std::min(
-                p[i].tile_a > 0 ? p[i].tile_a : default_tile(p[i].nsrc, NZ), NZ);
+            const int tile = std::min(batch_tile(p, n, p[i]), NZ);
             dmax = std::max(dmax, ceil_div(NZ, tile) + ceil_div(p[i].nx + 1, kLanesB) +
                                       ceil_div(p[i].ny + 1, kLanesC) - 2);
         }
@@ -1382,7 +1393,9 @@ size_t batch3_bytes(const fteik_problem *probs, int nprob) {
 
 int batch3_prepare(const fteik_problem *P, int n, int nsweep, int max_sets, int want_grad,
                    int group, char *ws, Batch3 **out, int *failed) {
-    if (group == 0) group = 2;
+    // one source per workgroup by default in a batch: measured on MI355X on the 48-model
+    // workload, 0.91 s against 0.94 s with two (and batch_tile's brick heights)
+    if (group == 0) group = 1;
     if (group != 1 && group != 2 && group != 4 && group != 8) return FTEIK_ERR_ARG;
     Batch3 *b = new Batch3;
     batch3_layout(P, n, &b->L);
@@ -1411,7 +1424,8 @@ int batch3_prepare(const fteik_problem *P, int n, int nsweep, int max_sets, int 
             }
         SweepArgs a;
         Ws3 w;
-        make_args(q.slow, q.nz, q.nx, q.ny, q.dz, q.dx, q.dy, q.nsrc, q.tile_a, group,
+        const int tile = batch_tile(P, n, q);
+        make_args(q.slow, q.nz, q.nx, q.ny, q.dz, q.dx, q.dy, q.nsrc, tile, group,
                   want_grad ? q.sgn : nullptr, q.tt, q.sets_done, ws + b->L.model_off[k], &a, &w);
         a.nlist = w.nlist = nlist_dev + k;
         if ((long long)a.ntiles * a.gmax >= (1LL << 31)) {

@@ -213,7 +213,8 @@ typedef struct fteik_problem {
     const double *slow;   /* device [nz, nx(, ny)] slowness */
     const double *src;    /* host [nsrc, ndim] sources relative to the model's origin */
     int nsrc;
-    int tile_a;           /* 3D brick height along z (0 = library default for nsrc) */
+    int tile_a;           /* 3D brick height along z (0 = the library default for the
+                             batch's total number of 3D sources, see fteik3d_solve) */
     double *tt;           /* device [nsrc, nz+1, nx+1(, ny+1)] traveltimes (output) */
     uint8_t *sgn;         /* device upwind signs, shaped as tt (uint8); NULL without want_grad */
     double *vzero;        /* device [nsrc] (output) */
@@ -237,7 +238,8 @@ size_t fteik_batch_workspace_size(const fteik_problem *probs, int nprob);
  *
  * nsweep, max_sets  as fteik3d_solve (nsweep <= 0: converge, at most max_sets sets)
  * want_grad         record upwind signs: every problem must then pass `sgn`
- * group             3D sources per workgroup (0 = default; 1, 2, 4, 8), the same for all
+ * group             3D sources per workgroup (0 = default, 1 in a batch; 1, 2, 4, 8), the
+ *                   same for all
  * failed            host out (may be NULL): index of the first problem with a source out
  *                   of bound, else -1
  * Checks every source of every problem before enqueuing anything (returns 1, "source out
