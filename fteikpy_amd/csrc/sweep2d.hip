@@ -4,8 +4,11 @@
 // 4-direction sweep set (_fteik2d.py:190-287).
 //
 // The reference's 2D set is one serial dependency chain: column j's +z pass reads column
-// j-1 after its -z pass.  Exact parity at any nsweep therefore needs the reference's
-// order, and each source is swept by one lane (the only lane of its wave); sources run in
+// j-1 after its -z pass, and within a pass each node reads the one before.  Exact parity
+// at any nsweep needs the results of that order: each source is swept by one wave, which
+// stages the two active columns in LDS and evaluates each z pass in speculative 64-node
+// blocks that keep only provably exact results, finishing dense stretches in order on one
+// lane (models too deep for LDS sweep in order in global memory); sources run in
 // parallel.
 #include "fteik_amd.h"
 #include "fteik_device.hpp"
