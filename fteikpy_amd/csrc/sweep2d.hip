@@ -8,7 +8,7 @@
 // at any nsweep needs the results of that order: each source is swept by one wave, which
 // stages the two active columns in LDS and evaluates each z pass in speculative 64-node
 // blocks that keep only provably exact results, finishing dense stretches in order on one
-// lane (models too deep for LDS sweep in order in global memory); sources run in
+// lane (models too deep for LDS stage the same rows in global memory); sources run in
 // parallel.
 #include "fteik_amd.h"
 #include "fteik_device.hpp"
@@ -155,23 +155,6 @@ __device__ __forceinline__ void record2(const Grid2 &G, int i, int j, int sg, in
     if (sg == 1) G.setsgn(i, j, sgntz, 0);
     else if (sg == 2) G.setsgn(i, j, 0, sgntx);
     else if (sg == 3) G.setsgn(i, j, sgntz, sgntx);
-}
-
-// 2D node update on the grid in global memory.  Returns true if the node changed.
-__device__ inline bool node2(const Grid2 &G, const SrcParam &P, double vzero, int i, int j,
-                             int sgnvz, int sgnvx, int sgntz, int sgntx) {
-    const int i1 = i - sgnvz, j1 = j - sgnvx;
-    const double tv = G.T(i - sgntz, j), te = G.T(i, j - sgntx);
-    const double tev = G.T(i - sgntz, j - sgntx), vref = G.S(i1, j1);
-    const Node2Pre q = node2_pre(G, te, tev, G.S(i1, max(j - 1, 0)), G.S(i1, min(j, G.NX - 2)),
-                                 G.S(max(i - 1, 0), j1), G.S(min(i, G.NZ - 2), j1), vref);
-    const bool far = abs(i - P.zsi) > kEpsIn || abs(j - P.xsi) > kEpsIn;
-    int sg;
-    const double tnew = node2_core(G, P, vzero, i, j, far, sgntz, sgntx, tv, te, tev, G.T(i, j),
-                                   q, vref, &sg);
-    G.T(i, j) = tnew;
-    record2(G, i, j, sg, sgntz, sgntx);
-    return sg != 0;
 }
 
 // Staged layout of the 2D sweep in LDS: rows r = 0 .. NZ of kRow2 doubles, row r for node
@@ -422,13 +405,15 @@ __global__ void fill2d_kernel(double *tt, long long n, uint8_t *sgn) {
     }
 }
 
-// Source initialisation (_fteik2d.py:355-617) and the sweep sets of source s.  lds null:
-// lane 0 does everything on the grid in global memory; else lds holds lds2_bytes(nz) and
-// the sets run staged (sets2_lds) by the whole wave.
+// Source initialisation (_fteik2d.py:355-617, by lane 0) and the sweep sets of source s
+// (sets2_lds, by the whole wave) with lds2_bytes(nz) of staged rows at `rows`: the
+// workgroup's LDS, or for a model too deep for it a per-source area of global memory (the
+// same code; each caller passes one kind, so the accesses keep their address space).
 __device__ __forceinline__ void solve_source2(const double *slow, int nz, int nx, double dz, double dx,
                               const SrcParam &P, int s, int nsweep, int max_sets,
                               double *tt_all, uint8_t *sgn_all, double *vzero_out,
                               int *sets_done, double *lds) {
+    // lds: the staged rows (see above)
     Grid2 G;
     G.slow = slow;
     G.nz = nz;
@@ -570,49 +555,33 @@ __device__ __forceinline__ void solve_source2(const double *slow, int nz, int nx
     G.ssum = G.dz2i + G.dx2i;
     G.rsum = 1.0 / G.ssum;
     const int cap = nsweep > 0 ? nsweep : max_sets;
-    if (lds) {
-        __syncthreads();  // lane 0's initialisation before the wave reads column 0
-        const int done = G.sgn ? sets2_lds<true>(G, P, vzero, cap, lds)
-                               : sets2_lds<false>(G, P, vzero, cap, lds);
-        if (threadIdx.x == 0) {
-            vzero_out[s] = vzero;
-            sets_done[s] = done;
-        }
-        return;
+    __syncthreads();  // lane 0's initialisation before the wave reads column 0
+    const int done = G.sgn ? sets2_lds<true>(G, P, vzero, cap, lds)
+                           : sets2_lds<false>(G, P, vzero, cap, lds);
+    if (threadIdx.x == 0) {
+        vzero_out[s] = vzero;
+        sets_done[s] = done;
     }
-    if (threadIdx.x != 0) return;
-    vzero_out[s] = vzero;
-    int done = 0;
-    for (int set = 0; set < cap; ++set) {
-        bool ch = false;
-        for (int j = 1; j < NX; ++j) {
-            for (int i = 1; i < NZ; ++i) ch |= node2(G, P, vzero, i, j, 1, 1, 1, 1);
-            for (int i = NZ - 2; i >= 0; --i) ch |= node2(G, P, vzero, i, j, 0, 1, -1, 1);
-        }
-        for (int j = NX - 2; j >= 0; --j) {
-            for (int i = 1; i < NZ; ++i) ch |= node2(G, P, vzero, i, j, 1, 0, 1, -1);
-            for (int i = NZ - 2; i >= 0; --i) ch |= node2(G, P, vzero, i, j, 0, 0, -1, -1);
-        }
-        ++done;
-        // a set that changed nothing leaves every later set a no-op
-        if (!ch) break;
-    }
-    sets_done[s] = done;
 }
 
-// One 64-lane wave per source (block s); staged: the model fits the launch's dynamic LDS
-// (lds2_bytes).  The bound lets the compiler keep the whole per-source chain in registers.
+// One 64-lane wave per source (block s).  scratch null: the staged rows fit the launch's
+// dynamic LDS; else scratch holds lds2_bytes(nz) per source in global memory.  The bound
+// lets the compiler keep the whole per-source chain in registers.
 __global__ __launch_bounds__(64) void fteik2d_kernel(const double *slow, int nz, int nx, double dz,
                                                      double dx,
                                const SrcParam *prm, int nsrc, int nsweep, int max_sets,
                                double *tt_all, uint8_t *sgn_all, double *vzero_out,
-                               int *sets_done, int staged) {
+                               int *sets_done, double *scratch) {
     extern __shared__ double lds2[];
     const int s = blockIdx.x;
     if (s >= nsrc) return;
     const SrcParam P = prm[s];  // in registers: the chain's stores cannot alias it
-    solve_source2(slow, nz, nx, dz, dx, P, s, nsweep, max_sets, tt_all, sgn_all, vzero_out,
-                  sets_done, staged ? lds2 : nullptr);
+    if (!scratch)
+        solve_source2(slow, nz, nx, dz, dx, P, s, nsweep, max_sets, tt_all, sgn_all, vzero_out,
+                      sets_done, lds2);
+    else
+        solve_source2(slow, nz, nx, dz, dx, P, s, nsweep, max_sets, tt_all, sgn_all, vzero_out,
+                      sets_done, scratch + (long long)s * (lds2_bytes(nz) / sizeof(double)));
 }
 
 // Batch of 2D models: block t takes source t - cum[m] of the model m with
@@ -626,6 +595,7 @@ struct Prob2 {
     uint8_t *sgn;
     double *vzero;
     int *sets_done;
+    double *scratch;  // staged rows of the model's sources in global memory, or null (LDS)
 };
 // The kernel's view of the same table: every pointer addresses device global memory
 // (address space 1), which lets the inlined chain use global instead of flat instructions.
@@ -639,13 +609,13 @@ struct Prob2Dev {
     FTEIK_GLOBAL uint8_t *sgn;
     FTEIK_GLOBAL double *vzero;
     FTEIK_GLOBAL int *sets_done;
+    FTEIK_GLOBAL double *scratch;
 };
 static_assert(sizeof(Prob2Dev) == sizeof(Prob2), "same table layout");
 
-// lds_bytes: the launch's dynamic LDS; models that fit it run staged.
+// Models with a scratch area stage their rows there, the others in the launch's LDS.
 __global__ __launch_bounds__(64) void fteik2d_batch_kernel(const Prob2 *probs, const int *cum,
-                                                           int nmodel, int nsweep, int max_sets,
-                                                           long long lds_bytes) {
+                                                           int nmodel, int nsweep, int max_sets) {
     extern __shared__ double lds2[];
     const int t = blockIdx.x;
     if (t >= cum[nmodel]) return;
@@ -658,9 +628,14 @@ __global__ __launch_bounds__(64) void fteik2d_batch_kernel(const Prob2 *probs, c
     const Prob2Dev Q = reinterpret_cast<const Prob2Dev *>(probs)[lo];
     const int s = t - cum[lo];
     const SrcParam P = ((const SrcParam *)Q.prm)[s];  // read once, before the chain
-    solve_source2((const double *)Q.slow, Q.nz, Q.nx, Q.dz, Q.dx, P, s, nsweep, max_sets,
-                  (double *)Q.tt, (uint8_t *)Q.sgn, (double *)Q.vzero, (int *)Q.sets_done,
-                  lds2_bytes(Q.nz) <= lds_bytes ? lds2 : nullptr);
+    if (!Q.scratch)
+        solve_source2((const double *)Q.slow, Q.nz, Q.nx, Q.dz, Q.dx, P, s, nsweep, max_sets,
+                      (double *)Q.tt, (uint8_t *)Q.sgn, (double *)Q.vzero, (int *)Q.sets_done,
+                      lds2);
+    else
+        solve_source2((const double *)Q.slow, Q.nz, Q.nx, Q.dz, Q.dx, P, s, nsweep, max_sets,
+                      (double *)Q.tt, (uint8_t *)Q.sgn, (double *)Q.vzero, (int *)Q.sets_done,
+                      (double *)(Q.scratch + (long long)s * (lds2_bytes(Q.nz) / sizeof(double))));
 }
 
 // 2D gradient post-pass, _fteik2d.py:623-639 (seeded analytic components survive where
@@ -773,14 +748,22 @@ extern "C" int fteik2d_solve(const double *slow, int nz, int nx, double dz, doub
     hipLaunchKernelGGL(fill2d_kernel, dim3(blocks), dim3(256), 0, stream, tt, total,
                        want_grad ? sgn : (uint8_t *)nullptr);
     FTEIK_CHECK(hipGetLastError());
-    // one wave per source: its lane 0 runs the serial chain (lanes of one wave would
-    // execute the union of the branches their nodes take), the others move columns
+    // one wave per source (lanes of one wave running different sources would execute the
+    // union of the branches their nodes take); the staged rows in LDS, or for a model too
+    // deep for it in stream-ordered scratch memory
     const long long lds = lds2_bytes(nz);
-    const int staged = lds <= lds2_limit();
-    hipLaunchKernelGGL(fteik2d_kernel, dim3(nsrc), dim3(64), staged ? (unsigned)lds : 0u, stream,
-                       slow, nz, nx, dz, dx, dprm, nsrc, nsweep, max_sets, tt,
-                       want_grad ? sgn : nullptr, vzero, sets_done, staged);
-    return hip_status(hipGetLastError());
+    double *scratch = nullptr;
+    if (lds > lds2_limit())
+        FTEIK_CHECK(hipMallocAsync((void **)&scratch, (size_t)lds * nsrc, stream));
+    hipLaunchKernelGGL(fteik2d_kernel, dim3(nsrc), dim3(64), scratch ? 0u : (unsigned)lds,
+                       stream, slow, nz, nx, dz, dx, dprm, nsrc, nsweep, max_sets, tt,
+                       want_grad ? sgn : nullptr, vzero, sets_done, scratch);
+    const int st = hip_status(hipGetLastError());
+    if (scratch) {
+        const int sf = hip_status(hipFreeAsync(scratch, stream));
+        return st ? st : sf;
+    }
+    return st;
 }
 
 extern "C" int fteik2d_gradient(const double *tt, const uint8_t *sgn, int nz, int nx, double dz,
@@ -882,6 +865,7 @@ int batch2_prepare(const fteik_problem *P, int n, int nsweep, int max_sets, int 
         p2.sgn = want_grad ? q.sgn : nullptr;
         p2.vzero = q.vzero;
         p2.sets_done = q.sets_done;
+        p2.scratch = nullptr;  // set at enqueue
         b->probs.push_back(p2);
         b->cum.push_back(b->cum.back() + q.nsrc);
         b->src.push_back(&q);
@@ -895,36 +879,56 @@ void batch2_free(Batch2 *b) { delete b; }
 int batch2_enqueue(Batch2 *b, hipStream_t stream) {
     const int n2 = (int)b->probs.size();
     if (n2 == 0) return 0;
-    FTEIK_CHECK(hipMemcpyAsync(b->base + b->off_probs, b->probs.data(), sizeof(Prob2) * n2,
-                               hipMemcpyHostToDevice, stream));
-    FTEIK_CHECK(hipMemcpyAsync(b->base + b->off_cum, b->cum.data(), sizeof(int) * (n2 + 1),
-                               hipMemcpyHostToDevice, stream));
-    FTEIK_CHECK(hipMemcpyAsync(b->base + b->off_prm, b->prm.data(),
-                               sizeof(SrcParam) * b->prm.size(), hipMemcpyHostToDevice, stream));
-    // the host arrays are freed by the caller after this returns: finish the copies
-    FTEIK_CHECK(hipStreamSynchronize(stream));
+    // Staged rows: in the launch's dynamic LDS (sized for the largest model that fits the
+    // device's per-workgroup LDS) or, for models too deep for it, in one stream-ordered
+    // scratch allocation carved per model
+    long long lds = 0, deep = 0;
+    const long long cap = lds2_limit();
     for (int m = 0; m < n2; ++m) {
+        const long long l = lds2_bytes(b->probs[m].nz);
+        if (l <= cap) lds = std::max(lds, l);
+        else deep += l * b->src[m]->nsrc;
+    }
+    char *scratch = nullptr;
+    if (deep) FTEIK_CHECK(hipMallocAsync((void **)&scratch, (size_t)deep, stream));
+    long long off = 0;
+    for (int m = 0; m < n2; ++m) {
+        const long long l = lds2_bytes(b->probs[m].nz);
+        b->probs[m].scratch = l <= cap ? nullptr : (double *)(scratch + off);
+        if (l > cap) off += l * b->src[m]->nsrc;
+    }
+    int st = hip_status(hipMemcpyAsync(b->base + b->off_probs, b->probs.data(),
+                                       sizeof(Prob2) * n2, hipMemcpyHostToDevice, stream));
+    if (!st)
+        st = hip_status(hipMemcpyAsync(b->base + b->off_cum, b->cum.data(),
+                                       sizeof(int) * (n2 + 1), hipMemcpyHostToDevice, stream));
+    if (!st)
+        st = hip_status(hipMemcpyAsync(b->base + b->off_prm, b->prm.data(),
+                                       sizeof(SrcParam) * b->prm.size(), hipMemcpyHostToDevice,
+                                       stream));
+    // the host arrays are freed by the caller after this returns: finish the copies
+    if (!st) st = hip_status(hipStreamSynchronize(stream));
+    for (int m = 0; m < n2 && !st; ++m) {
         const fteik_problem &q = *b->src[m];
         const long long total = (long long)(q.nz + 1) * (q.nx + 1) * q.nsrc;
         const int blocks = (int)std::min<long long>((total + 255) / 256, 65536);
         hipLaunchKernelGGL(fill2d_kernel, dim3(blocks), dim3(256), 0, stream, q.tt, total,
                            b->grad ? q.sgn : (uint8_t *)nullptr);
-        FTEIK_CHECK(hipGetLastError());
+        st = hip_status(hipGetLastError());
     }
-    // one wave per source, its lane 0 running the chain: sources of different models (and
-    // different branches of the node update) would otherwise serialise each other inside a
-    // wave; measured 2x faster on the uneven batch.  Dynamic LDS: the largest model that
-    // fits the device's per-workgroup LDS; larger ones run on global memory.
-    long long lds = 0;
-    const long long cap = lds2_limit();
-    for (int m = 0; m < n2; ++m) {
-        const long long l = lds2_bytes(b->probs[m].nz);
-        if (l <= cap && l > lds) lds = l;
+    // one wave per source: sources of different models (and different branches of the node
+    // update) would otherwise serialise each other inside a wave
+    if (!st) {
+        hipLaunchKernelGGL(fteik2d_batch_kernel, dim3(b->cum.back()), dim3(64), (unsigned)lds,
+                           stream, (const Prob2 *)(b->base + b->off_probs),
+                           (const int *)(b->base + b->off_cum), n2, b->nsweep, b->max_sets);
+        st = hip_status(hipGetLastError());
     }
-    hipLaunchKernelGGL(fteik2d_batch_kernel, dim3(b->cum.back()), dim3(64), (unsigned)lds,
-                       stream, (const Prob2 *)(b->base + b->off_probs),
-                       (const int *)(b->base + b->off_cum), n2, b->nsweep, b->max_sets, lds);
-    return hip_status(hipGetLastError());
+    if (scratch) {
+        const int sf = hip_status(hipFreeAsync(scratch, stream));
+        if (!st) st = sf;
+    }
+    return st;
 }
 
 }  // namespace fteik

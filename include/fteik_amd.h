@@ -184,10 +184,10 @@ size_t fteik2d_workspace_size(int nsrc);
  * fteik3d_solve with src [nsrc, 2] and tt [nsrc, nz+1, nx+1].  The reference's 2D set is
  * a single serial dependency chain (columns interleave the +z and -z passes), so each
  * source gets the results of the reference's exact order; sources run in parallel, one
- * wave each.  Models whose (nz+2) x 72 B fit one workgroup's LDS (nz <= 2273 on MI355X)
- * stage the two active columns in LDS and evaluate each z pass in speculative blocks of 64
- * nodes that finalize only provably exact nodes; deeper ones sweep in order in global
- * memory, with the same results.  dz, dx in [1e-100, 1e100]
+ * wave each, which stages the two active columns ((nz+2) x 72 B) and evaluates each z pass
+ * in speculative blocks of 64 nodes that finalize only provably exact nodes.  The staged
+ * rows sit in LDS when they fit one workgroup's (nz <= 2273 on MI355X); deeper models get
+ * them in global memory the call allocates stream-ordered (hipMallocAsync) and frees.  dz, dx in [1e-100, 1e100]
  * (else FTEIK_ERR_ARG), as for fteik3d_solve.
  */
 int fteik2d_solve(const double *slow, int nz, int nx, double dz, double dx, const double *src,

@@ -227,7 +227,8 @@ def test_2d_speculative_columns_bitwise(fa):
 
 def test_2d_deeper_than_lds_bitwise(fa):
     """A model deeper than one workgroup's LDS holds (2300 cells: 2302 staged rows of 72 B
-    exceed 160 KiB) sweeps on the grid in global memory, with the same results."""
+    exceed 160 KiB) stages its rows in global memory instead, with the same results; in a
+    batch beside a model staged in LDS, too."""
     v = _rough_model((2300, 12), 22)
     d = (5.0, 6.0)
     srcs = np.array([[0.4, 0.5], [0.95, 0.1]]) * (np.array(v.shape) * d)
@@ -235,6 +236,12 @@ def test_2d_deeper_than_lds_bitwise(fa):
         ref, _, _, k = oracle.solve2d(1.0 / v, *d, srcs, nsweep=ns or 0)
         tts = fa.Eikonal2D(v, d).solve(srcs, nsweep=ns)
         assert np.array_equal(np.stack([t.grid for t in tts]), ref), ns
+    vs = _rough_model((50, 40), 23)
+    ss = np.array([[0.3, 0.6]]) * (np.array(vs.shape) * d)
+    out = fa.solve_batch_tensors([(fa.Eikonal2D(vs, d), ss), (fa.Eikonal2D(v, d), srcs)],
+                                 nsweep=None)
+    assert np.array_equal(out[0][0].cpu().numpy(), oracle.solve2d(1.0 / vs, *d, ss, nsweep=0)[0])
+    assert np.array_equal(out[1][0].cpu().numpy(), oracle.solve2d(1.0 / v, *d, srcs, nsweep=0)[0])
 
 
 @pytest.mark.parametrize("case", range(12))
