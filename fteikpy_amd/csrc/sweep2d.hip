@@ -78,7 +78,7 @@ __device__ __forceinline__ double delta(double t1, double tauv, double taue, dou
 // the diagonal; sza, szb the cells of the z 1D operator (slow[i1, max(j-1, 0)] and
 // slow[i1, min(j, nx-1)]); sxa, sxb those of the x 1D operator (slow[max(i-1, 0), j1] and
 // slow[min(i, nz-1), j1]); vref the 2D operator's cell slow[i1, j1].  Each value is the
-// reference's own subexpression, evaluated in its order.
+// reference's own subexpression, evaluated in its order (stage2 computes them).
 struct Node2Pre {
     double cz;    // dz * min(sza, szb): t1d1 = tv + cz
     double t1d2;  // te + dx * min(sxa, sxb)
@@ -89,22 +89,11 @@ struct Node2Pre {
 
 __device__ __forceinline__ double nan2() { return __builtin_nan(""); }
 
-__device__ __forceinline__ Node2Pre node2_pre(const Grid2 &G, double te, double tev, double sza,
-                                              double szb, double sxa, double sxb, double vref) {
-    Node2Pre q;
-    q.cz = G.dz * pymin(sza, szb);
-    q.t1d2 = te + G.dx * pymin(sxa, sxb);
-    q.A = te >= tev ? te + G.dx * vref : nan2();
-    q.B = G.dz * vref;
-    q.K = 4.0 * (vref * vref) * G.ssum;
-    return q;
-}
-
 // 2D node update, _fteik2d.py:70-187: tv, te, tev the upwind neighbours, t0 the node's
-// current time, q = node2_pre(...), vref the 2D operator's cell; far: the node is more than
-// 5 nodes from the source cell along z or x (else the spherical operator applies).  Returns
-// the new time; *sg receives which upwind signs to record: 0 unchanged, 1 z only, 2 x only,
-// 3 both.
+// current time, q its Node2Pre (computed by stage2), vref the 2D operator's cell; far: the
+// node is more than 5 nodes from the source cell along z or x (else the spherical operator
+// applies).  Returns the new time; *sg receives which upwind signs to record: 0 unchanged,
+// 1 z only, 2 x only, 3 both.
 __device__ __forceinline__ double node2_core(const Grid2 &G, const SrcParam &P, double vzero,
                                              int i, int j, bool far, int sgntz, int sgntx,
                                              double tv, double te, double tev, double t0,
