@@ -147,8 +147,8 @@ __device__ __forceinline__ void record2(const Grid2 &G, int i, int j, int sg, in
 }
 
 // Staged layout of the 2D sweep (in LDS, or in global memory for a model too deep for it):
-// rows r = 0 .. NZ of kRow2 doubles, row r for node
-// i = r - 1 and cell k = clamp(r - 1, 0, nz - 1) of the current column j:
+// rows r = 0 .. NZ of kRow2 doubles, row r for node i = r - 1 and cell
+// k = clamp(r - 1, 0, nz - 1) of the current column j:
 //   0, 1  node i of the previous and of the current node column (roles swap per column)
 //   2     dz * min(slow[k, max(j-1, 0)], slow[k, min(j, nx-1)])   (Node2Pre::cz)
 //   3     slow[k, j1], j1 = j - sgnvx                            (vref)
@@ -348,9 +348,8 @@ __device__ __forceinline__ int column2_pv(const Grid2 &G, const SrcParam &P, dou
 }
 
 // Sweep sets of one source with the two active node columns staged at `lds` (LDS or
-// global scratch)
-// (lds2_bytes(nz)), in the reference's order, _fteik2d.py:190-287, with the results of
-// that order (column2).  The wave's 64 lanes move columns between the grid and the rows and
+// global scratch, lds2_bytes(nz)), in the reference's order, _fteik2d.py:190-287, with the
+// results of that order (column2).  The wave's 64 lanes move columns between the grid and the rows and
 // precompute the values that do not depend on the serial chain.  Returns the number of
 // sets run.
 template <bool GRAD>
