@@ -327,7 +327,7 @@ def plain_2d(torch, fa, synthetic, dev, with_cpu=True, reps=3):
     nsweep=2 and converged, then traveltimes at 200 receivers; GPU wall time (best of
     `reps`) beside the oracle's single-thread C restatement of the reference's numba path
     on the same solve (the 2D chain is serial per source, so one source is latency bound on
-    either side)."""
+    either side); then 512 sources converged, beside the oracle on the host's threads."""
     v, d, src, rec = synthetic.model_2d_gaussians()
     eik = fa.Eikonal2D(v, d)
     out = {}
@@ -347,6 +347,24 @@ def plain_2d(torch, fa, synthetic, dev, with_cpu=True, reps=3):
             oracle.solve2d(1.0 / v, *d, src[None, :], nsweep=ns or 0, grad=True, nthreads=1)
             entry["cpu_oracle_1_thread_s"] = time.perf_counter() - tc
         out[label] = entry
+    # many sources: 512 uniform in the model, converged
+    srcs = np.random.default_rng(3).random((512, 2)) * np.array(v.shape) * d
+    ts = []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        eik.solve(srcs, nsweep=None)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    entry = {"gpu_s": min(ts[1:])}
+    if with_cpu:
+        import oracle
+
+        ncores = min(os.cpu_count() or 1, 16)
+        tc = time.perf_counter()
+        oracle.solve2d(1.0 / v, *d, srcs, nsweep=0, nthreads=ncores)
+        entry["cpu_oracle_s"] = time.perf_counter() - tc
+        entry["cpu_threads"] = ncores
+    out["converged_512_sources"] = entry
     return out
 
 
