@@ -191,7 +191,7 @@ __device__ __forceinline__ Ops2 ops2(const double *lds, int i) {
 // lane: the upwind z neighbour tv is the previous update's result, kept in a register, and
 // the next node's operands are read before the current update.  tv: in, the final time of
 // the node before i; out, that of the last node.  Returns nonzero if a node changed.
-template <int SX, int PV, bool GRAD, int DZ>
+template <int SX, int PV, bool GRAD, int DZ, bool NEAR = true>
 __device__ __forceinline__ int serial2(const Grid2 &G, const SrcParam &P, double vzero, int j,
                                        int nlo, double *lds, int i, int cnt, double &tv) {
     constexpr int CU = 1 - PV;
@@ -200,7 +200,8 @@ __device__ __forceinline__ int serial2(const Grid2 &G, const SrcParam &P, double
 #pragma unroll 2
     for (int k = 0; k < cnt; ++k, i += DZ) {
         const Ops2 on = ops2<DZ, PV>(lds, k + 1 < cnt ? i + DZ : i);  // the last reads itself
-        const bool far = (unsigned)(i - nlo) > 2u * kEpsIn;
+        // NEAR false: the caller has checked that no node of the run is near the source
+        const bool far = !NEAR || (unsigned)(i - nlo) > 2u * kEpsIn;
         const double tnew = node2_core(G, P, vzero, i, j, far, DZ, SX, tv, o.te, o.tev, o.t0,
                                        o.q, o.vref, &sg);
         lds[kRow2 * (i + 1) + CU] = tnew;
@@ -280,8 +281,13 @@ __device__ __forceinline__ int column2(const Grid2 &G, const SrcParam &P, double
         const int n = min(64, NZ - i0);
         int e;
         ch |= spec2<SX, PV, GRAD, 1>(G, P, vzero, j, nlo, lds, i0, n, tv, &e);
-        if (e < n && threadIdx.x == 0)
-            ch |= serial2<SX, PV, GRAD, 1>(G, P, vzero, j, nlo, lds, i0 + e, n - e, tv);
+        if (e < n && threadIdx.x == 0) {
+            const int a = i0 + e, b = i0 + n - 1;  // rows of the run, ascending
+            if (b < nlo || a > nlo + 2 * kEpsIn)
+                ch |= serial2<SX, PV, GRAD, 1, false>(G, P, vzero, j, nlo, lds, a, n - e, tv);
+            else
+                ch |= serial2<SX, PV, GRAD, 1>(G, P, vzero, j, nlo, lds, a, n - e, tv);
+        }
         tv = __shfl(tv, 0);
         __syncthreads();
         i0 += n;
@@ -290,8 +296,14 @@ __device__ __forceinline__ int column2(const Grid2 &G, const SrcParam &P, double
         const int n = min(64, i0 + 1);
         int e;
         ch |= spec2<SX, PV, GRAD, -1>(G, P, vzero, j, nlo, lds, i0, n, tv, &e);
-        if (e < n && threadIdx.x == 0)
-            ch |= serial2<SX, PV, GRAD, -1>(G, P, vzero, j, nlo, lds, i0 - e, n - e, tv);
+        if (e < n && threadIdx.x == 0) {
+            const int a = i0 - (n - 1), b = i0 - e;  // rows of the run, ascending
+            if (b < nlo || a > nlo + 2 * kEpsIn)
+                ch |= serial2<SX, PV, GRAD, -1, false>(G, P, vzero, j, nlo, lds, i0 - e, n - e,
+                                                       tv);
+            else
+                ch |= serial2<SX, PV, GRAD, -1>(G, P, vzero, j, nlo, lds, i0 - e, n - e, tv);
+        }
         tv = __shfl(tv, 0);
         __syncthreads();
         i0 -= n;
