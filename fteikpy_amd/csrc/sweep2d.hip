@@ -226,7 +226,7 @@ __device__ __forceinline__ int serial2(const Grid2 &G, const SrcParam &P, double
 // finalized in an iteration (a dense dependency chain, cheaper on one lane).  The exact
 // prefix is written back; returns its length in *done and its last time in tv, and
 // nonzero on lanes whose node changed.
-template <int SX, int PV, bool GRAD, int DZ>
+template <int SX, int PV, bool GRAD, int DZ, bool NEAR = true>
 __device__ __forceinline__ int spec2(const Grid2 &G, const SrcParam &P, double vzero, int j,
                                      int nlo, double *lds, int i0, int n, double &tv,
                                      int *done) {
@@ -237,7 +237,8 @@ __device__ __forceinline__ int spec2(const Grid2 &G, const SrcParam &P, double v
     // current time of the upwind node (+z: node i-1, row i; -z: node i+1, row i+2)
     const double up = lds[R * (DZ > 0 ? i : i + 2) + CU];
     double g = lane == 0 ? tv : up, yp = o.t0, y;
-    const bool far = (unsigned)(i - nlo) > 2u * kEpsIn;
+    // NEAR false: the caller has checked that no node of the block is near the source
+    const bool far = !NEAR || (unsigned)(i - nlo) > 2u * kEpsIn;
     int e = 0, sg;
     while (true) {
         y = node2_core(G, P, vzero, i, j, far, DZ, SX, g, o.te, o.tev, o.t0, o.q, o.vref, &sg);
@@ -280,7 +281,10 @@ __device__ __forceinline__ int column2(const Grid2 &G, const SrcParam &P, double
     for (int i0 = 1; i0 < NZ;) {  // +z: nodes 1 .. NZ-1
         const int n = min(64, NZ - i0);
         int e;
-        ch |= spec2<SX, PV, GRAD, 1>(G, P, vzero, j, nlo, lds, i0, n, tv, &e);
+        if (i0 + n - 1 < nlo || i0 > nlo + 2 * kEpsIn)
+            ch |= spec2<SX, PV, GRAD, 1, false>(G, P, vzero, j, nlo, lds, i0, n, tv, &e);
+        else
+            ch |= spec2<SX, PV, GRAD, 1>(G, P, vzero, j, nlo, lds, i0, n, tv, &e);
         if (e < n && threadIdx.x == 0) {
             const int a = i0 + e, b = i0 + n - 1;  // rows of the run, ascending
             if (b < nlo || a > nlo + 2 * kEpsIn)
@@ -295,7 +299,10 @@ __device__ __forceinline__ int column2(const Grid2 &G, const SrcParam &P, double
     for (int i0 = NZ - 2; i0 >= 0;) {  // -z: nodes NZ-2 .. 0, tv = node NZ-1 from +z
         const int n = min(64, i0 + 1);
         int e;
-        ch |= spec2<SX, PV, GRAD, -1>(G, P, vzero, j, nlo, lds, i0, n, tv, &e);
+        if (i0 < nlo || i0 - (n - 1) > nlo + 2 * kEpsIn)
+            ch |= spec2<SX, PV, GRAD, -1, false>(G, P, vzero, j, nlo, lds, i0, n, tv, &e);
+        else
+            ch |= spec2<SX, PV, GRAD, -1>(G, P, vzero, j, nlo, lds, i0, n, tv, &e);
         if (e < n && threadIdx.x == 0) {
             const int a = i0 - (n - 1), b = i0 - e;  // rows of the run, ascending
             if (b < nlo || a > nlo + 2 * kEpsIn)
