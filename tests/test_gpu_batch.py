@@ -51,6 +51,25 @@ def test_batch_equals_single_solves_and_oracle(fa, nsweep):
                 assert sets[i] == k
 
 
+@pytest.mark.parametrize("nsweep", [2, None])
+def test_batch_2d_several_blocks(fa, nsweep):
+    """2D models 60 to 150 nodes deep (two and three 64-node speculative blocks per z pass)
+    beside a small 3D model: times and signs equal the single solves, times the oracle."""
+    models = synthetic.uneven_batch(seed=12, n2=4, n3=1, side2=(60, 150),
+                                    side3=((6, 10), (6, 10)), nsrc=(1, 4))
+    probs = _solvers(fa, models)
+    res = fa.solve_batch_tensors(probs, nsweep=nsweep, return_gradient=True)
+    for (eik, src), (v, d, _), (tt, sgn, vz, sets) in zip(probs, models, res):
+        if v.ndim == 3:
+            continue
+        tt1, sgn1, vz1, sets1 = _solve2d_tensors(eik, src, nsweep)
+        assert np.array_equal(tt.cpu().numpy(), tt1.cpu().numpy()), v.shape
+        assert np.array_equal(sgn.cpu().numpy(), sgn1.cpu().numpy()), v.shape
+        assert np.array_equal(sets, sets1), v.shape
+        ref = oracle.solve2d(1.0 / v, *d, src, nsweep=nsweep or 0)[0]
+        assert np.array_equal(tt.cpu().numpy(), ref), v.shape
+
+
 def _solve2d_tensors(eik, src, nsweep):
     grids = eik.solve(src, nsweep=nsweep, return_gradient=True)
     grids = grids if isinstance(grids, list) else [grids]
