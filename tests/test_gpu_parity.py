@@ -276,6 +276,40 @@ def test_2d_random_models_bitwise(fa, case):
                 assert t.sets_done == k[s], (case, s)
 
 
+def test_3d_random_models_bitwise(fa):
+    """20 random 3D models (1 to 39 cells per side, smooth to cell-scale contrast, 1-5
+    sources, some on a face) at random nsweep, brick height and sources per workgroup:
+    times and set counts equal the oracle's; the converged ones again as one batch with
+    the batch's default brick heights and group."""
+    probs, refs = [], []
+    for case in range(20):
+        rng = np.random.default_rng(7000 + case)
+        shape = tuple(int(x) for x in rng.integers(1, 40, 3))
+        p = rng.standard_normal(shape)
+        for _ in range(int(rng.integers(0, 3))):
+            for ax in range(3):
+                p = (np.roll(p, 1, ax) + p + np.roll(p, -1, ax)) / 3.0
+        p = p / max(np.abs(p).max(), 1e-12)
+        v = rng.uniform(1000, 4000) * np.exp(rng.uniform(0, 1.0) * p)
+        d = tuple(rng.uniform(0.5, 30.0, 3))
+        pos = rng.random((int(rng.integers(1, 6)), 3))
+        if rng.random() < 0.5:
+            pos[0, int(rng.integers(0, 3))] = float(rng.integers(0, 2))
+        srcs = pos * (np.array(shape) * d)
+        ns = [1, 2, 3, None][int(rng.integers(0, 4))]
+        ref, _, _, k = oracle.solve3d(1.0 / v, *d, srcs, nsweep=ns or 0)
+        eik = fa.Eikonal3D(v, d)
+        tile, group = int(rng.choice([0, 1, 5, 8, 16, 27])), int(rng.choice([0, 1, 2, 4, 8]))
+        tt, _, _, sets = eik.solve_tensors(srcs, nsweep=ns, tile_a=tile, group=group)
+        assert np.array_equal(tt.cpu().numpy(), ref), (case, shape, ns, tile, group)
+        if ns is None:
+            assert np.array_equal(sets, k), case
+            probs.append((eik, srcs))
+            refs.append(ref)
+    for (tt, _, _, _), ref in zip(fa.solve_batch_tensors(probs, nsweep=None), refs):
+        assert np.array_equal(tt.cpu().numpy(), ref)
+
+
 def test_zero_velocity_block_bitwise(fa):
     """Zero-velocity cells (infinite slowness, no check in the reference either): candidates
     through them come out infinite, or NaN from the fixed-divisor division where the
