@@ -139,11 +139,12 @@ __device__ __forceinline__ double node2_core(const Grid2 &G, const SrcParam &P, 
 }
 
 // Upwind signs of a changed node, _fteik2d.py:176-187.
+// Called only where signs are recorded (G.sgn set); sg's bit 0 = z, bit 1 = x take part.
 __device__ __forceinline__ void record2(const Grid2 &G, int i, int j, int sg, int sgntz,
                                         int sgntx) {
-    if (sg == 1) G.setsgn(i, j, sgntz, 0);
-    else if (sg == 2) G.setsgn(i, j, 0, sgntx);
-    else if (sg == 3) G.setsgn(i, j, sgntz, sgntx);
+    if (sg == 0) return;
+    const int sz = (sg & 1) ? sgntz : 0, sx = (sg & 2) ? sgntx : 0;
+    G.sgn[(long long)i * G.NX + j] = (uint8_t)(sgn_code(sz) | (sgn_code(sx) << 2));
 }
 
 // Staged layout of the 2D sweep (in LDS, or in global memory for a model too deep for it):
