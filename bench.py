@@ -145,6 +145,9 @@ def main():
 
     del out, tt
     torch.cuda.empty_cache()
+    with torch.cuda.stream(stream):
+        post = None if args.no_analytic else headline_post(torch, fteikpy_amd, eik, src, rec, d)
+    torch.cuda.empty_cache()
     # every section runs on the one stream above: HIP maps a process's streams onto at most
     # GPU_MAX_HW_QUEUES hardware queues (4 by default), and the multi-model batch overlaps
     # its 2D and 3D parts only while its side stream has a queue of its own
@@ -243,6 +246,7 @@ def main():
         "fwd_adjoint": adj,
         "analytic": ana,
         "single_source_200cubed": c2,
+        "headline_post": post,
         "uneven_batch": ub,
         "plain_2d_101x301": p2d,
     }
@@ -267,6 +271,34 @@ def copy_calibration(gib):
     dt = time.perf_counter() - t0
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "seconds": dt,
                       "checksum": float(y[::4096].sum())}), flush=True)
+
+
+def headline_post(torch, fa, eik, src, rec, d, nrays=64):
+    """The rest of the batched configuration: the converged solve again with gradient
+    signs, then per source its gradient grids, traveltimes at the 4096 receivers and
+    `nrays` ray paths (to the first receivers), one source after another (wall time)."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tt, sgn, vz, sets = eik.solve_tensors(src, nsweep=None, return_gradient=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    npts = 0
+    for i in range(len(src)):
+        g = fa.TraveltimeGrid3D(tt[i], d, np.zeros(3), src[i], None, float(vz[i]), sign=sgn[i])
+        g(rec)
+        rays = g.raytrace(rec[:nrays])
+        npts += sum(len(r) for r in rays)
+        del g
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    del tt, sgn
+    log(f"headline post: solve with signs {t1 - t0:.2f} s, receivers and rays {t2 - t1:.2f} s")
+    return {"solve_with_signs_s": t1 - t0, "receivers_and_rays_s": t2 - t1,
+            "receivers_per_source": len(rec), "rays_per_source": nrays,
+            "ray_points": npts,
+            "note": "per source: gradient grids on the device (from the signs), "
+                    "traveltimes at its receivers, rays to the first receivers (returned "
+                    "to the host as the reference's lists of arrays)"}
 
 
 def single_source_c2(torch, fa, synthetic, dev, reps=3):
