@@ -2,7 +2,7 @@
 
 ``receiver_traveltimes(slowness, gridsize, sources, receivers)`` returns the traveltimes
 T[s, r] that ``Eikonal3D(1 / slowness, gridsize, origin).solve(sources, nsweep=None)[s]
-(receivers[r])`` gives (fteikpy/_solver.py:120-126 then fteikpy/_grid.py:342-368: the
+(receivers[r])`` gives (fteikpy/_solver.py:120-126 then fteikpy/_grid.py:193-219: the
 converged 3D sweep and the apparent-velocity interpolation), as a torch tensor whose
 gradient with respect to ``slowness`` comes from hand-written adjoint kernels:
 

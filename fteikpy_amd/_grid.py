@@ -1,10 +1,14 @@
 """Device-resident grid objects with the reference's interface.
 
 Mirrors fteikpy/_base.py:10-241 (BaseGrid, BaseGrid2D/3D, BaseTraveltime) and
-fteikpy/_grid.py:8-427 (Grid2D/3D, TraveltimeGrid2D/3D).  Arrays live in HIP device memory
-as torch float64 tensors; ``.grid`` returns a host numpy copy (as the reference returns
-numpy) and ``.tensor`` the device tensor.  Interpolation, apparent-velocity traveltime
-interpolation, gradients and ray tracing run as HIP kernels through the C ABI.
+fteikpy/_grid.py:8-278 (Grid2D/3D, TraveltimeGrid2D/3D).  Arrays live in HIP device memory
+as torch float64 tensors.  ``.grid`` returns a host numpy array that is live as the
+reference's (fteikpy/_base.py:22-25): in-place writes to it are seen by the next device
+operation on the grid (solve, interpolation, resample, smooth, gradient, rays), which first
+uploads it.  ``.tensor`` returns the device tensor and makes it the live copy again: a
+later ``.grid`` copies it back, and writes to a numpy array handed out before are then no
+longer seen.  Interpolation, apparent-velocity traveltime interpolation, gradients and ray
+tracing run as HIP kernels through the C ABI.
 """
 import ctypes
 
@@ -57,6 +61,15 @@ def _element_stride(t):
     return k
 
 
+def _keep_gradient(grid):
+    """Before the times of a traveltime grid can change (resample(), smooth(), writes to a
+    handed-out .grid), materialise the gradient that belongs to the solved times: the
+    reference computes it at solve time and keeps it."""
+    keep = getattr(grid, "_keep_solve_gradient", None)
+    if keep is not None:
+        keep()
+
+
 class BaseGrid:
     """Grid storage and properties, fteikpy/_base.py:10-50."""
 
@@ -65,7 +78,7 @@ class BaseGrid:
     def __init__(self, grid, gridsize, origin, **kwargs):
         super().__init__(**kwargs)
         self._dev = to_device(grid)
-        self._host = None
+        self._host = None  # host array handed out by .grid, the live copy while not None
         self._gridsize = tuple(float(x) for x in gridsize)
         self._origin = np.asarray(origin, dtype=np.float64)
 
@@ -74,14 +87,25 @@ class BaseGrid:
 
     @property
     def grid(self):
-        """Grid values as a host numpy array (copied from the device once)."""
+        """Grid values as a live host numpy array (see the module docstring)."""
         if self._host is None:
+            _keep_gradient(self)  # writes to the times must not reach the solve's gradient
             self._host = self._dev.detach().cpu().numpy()
         return self._host
 
     @property
     def tensor(self):
-        """Grid values as a device tensor."""
+        """Grid values as a device tensor (from now on the live copy)."""
+        dev = self._device()
+        self._host = None
+        return dev
+
+    def _device(self):
+        """The device tensor, after uploading the handed-out host array if there is one."""
+        if self._host is not None:
+            if tuple(self._host.shape) != tuple(self._dev.shape):
+                raise ValueError("the grid array changed shape; assign a new grid instead")
+            self._dev.copy_(_torch().from_numpy(np.ascontiguousarray(self._host, np.float64)))
         return self._dev
 
     @property
@@ -109,10 +133,11 @@ class BaseGrid:
 
     def _strided(self):
         """(tensor, element stride) usable by the interpolation kernels."""
-        k = _element_stride(self._dev)
+        dev = self._device()
+        k = _element_stride(dev)
         if k is None:
-            return self._dev.contiguous(), 1
-        return self._dev, k
+            return dev.contiguous(), 1
+        return dev, k
 
     def resample(self, new_shape, method="linear"):
         """Resample the grid on the device (fteikpy/_base.py:81-112, :165-201): values at
@@ -127,7 +152,8 @@ class BaseGrid:
             raise ValueError(f"new_shape must have {nd} entries")
         if min(self.shape) < 2:
             raise ValueError("resampling needs at least 2 nodes per axis")
-        src = self._dev.contiguous()
+        _keep_gradient(self)
+        src = self._device().contiguous()
         out = torch.empty(new_shape, dtype=torch.float64, device=device())
         shp = (ctypes.c_int * nd)(*self.shape)
         nshp = (ctypes.c_int * nd)(*new_shape)
@@ -153,7 +179,8 @@ class BaseGrid:
         nd = self._ndim
         sigma = np.full(nd, sigma) if np.ndim(sigma) == 0 else np.asarray(sigma)
         sig = sigma / np.asarray(self._gridsize)
-        cur = self._dev.contiguous()
+        _keep_gradient(self)
+        cur = self._device().contiguous()
         shp = (ctypes.c_int * nd)(*self.shape)
         L = _lib.lib()
         for axis in range(nd):
@@ -264,6 +291,14 @@ class BaseTraveltime:
     def _rel_source(self):
         return self._source - self._origin
 
+    def _keep_solve_gradient(self):
+        """The gradient of the solved times (fteikpy/_base.py:230-236 stores it at
+        construction): computed from the sign grid now, before resample() or smooth()
+        replace the times it belongs to."""
+        if self._gradient_dev is None and self._sign is not None:
+            self._gradient_tensor()
+        self._sign = None
+
     def _gradient_tensor(self):
         if self._gradient_dev is None:
             if self._sign is None:
@@ -271,10 +306,14 @@ class BaseTraveltime:
                     "no gradient grid, use option `return_gradient` to return gradient grids")
             torch = _torch()
             nd = self._ndim
+            # the sign grid belongs to the solved times: resample() and smooth() keep the
+            # gradient materialised first, as the reference keeps its solve-time gradient
+            if tuple(self._sign.shape) != self.shape:
+                raise RuntimeError("sign grid does not match the traveltime grid")
             g = torch.empty(self.shape + (nd,), dtype=torch.float64, device=device())
             cells = [n - 1 for n in self.shape]
             src, sp = _lib.host_f8(self._rel_source())
-            tt = self._dev.contiguous()
+            tt = self._device().contiguous()
             L = _lib.lib()
             if nd == 3:
                 st = L.fteik3d_gradient(_ptr(tt), _ptr(self._sign), *cells, *self._gridsize, sp,
@@ -294,7 +333,7 @@ class BaseTraveltime:
         q2 = np.ascontiguousarray(q.reshape(-1, nd))
         qd = torch.as_tensor(q2, device=device())
         out = torch.empty(len(q2), dtype=torch.float64, device=device())
-        tt = self._dev.contiguous()
+        tt = self._device().contiguous()
         o, op = _lib.host_f8(self._origin)
         d, dp = _lib.host_f8(self._gridsize)
         s, sp = _lib.host_f8(self._source)
@@ -315,6 +354,9 @@ class BaseTraveltime:
         if not max_step:
             max_dist = 2.0 * sum((n * d) ** 2 for n, d in zip(self.shape, self._gridsize)) ** 0.5
             max_step = int(max_dist / stepsize)
+        if tuple(grad.shape[:-1]) != self.shape:
+            raise ValueError("the gradient grid belongs to the solved traveltime grid, which "
+                             "was resampled since: rays need the solved shape")
         p = np.asarray(points, dtype=np.float64)
         single = p.ndim == 1
         p2 = np.ascontiguousarray(p.reshape(-1, nd))
@@ -359,7 +401,7 @@ class Grid3D(BaseGrid3D):
 
 
 class TraveltimeGrid2D(BaseGrid2D, BaseTraveltime):
-    """2D traveltime grid, fteikpy/_grid.py:193-307."""
+    """2D traveltime grid, fteikpy/_grid.py:44-158."""
 
     def __init__(self, grid, gridsize, origin, source, gradient, vzero, sign=None, sets_done=None):
         super().__init__(grid=grid, gridsize=gridsize, origin=np.asarray(origin, dtype=np.float64),
@@ -367,22 +409,22 @@ class TraveltimeGrid2D(BaseGrid2D, BaseTraveltime):
                          sets_done=sets_done)
 
     def __call__(self, points, fill_value=np.nan):
-        """Bilinear apparent velocity interpolation (fteikpy/_grid.py:225-250)."""
+        """Bilinear apparent velocity interpolation (fteikpy/_grid.py:76-101)."""
         return self._vinterp(points, fill_value)
 
     def raytrace(self, points, stepsize=None, max_step=None, honor_grid=False):
-        """2D a posteriori ray-tracing (fteikpy/_grid.py:252-294)."""
+        """2D a posteriori ray-tracing (fteikpy/_grid.py:103-145)."""
         return self._raytrace(points, stepsize, max_step, honor_grid)
 
     @property
     def gradient(self):
-        """Z and X gradient grids as a list of Grid2D (fteikpy/_grid.py:296-307)."""
+        """Z and X gradient grids as a list of Grid2D (fteikpy/_grid.py:147-158)."""
         g = self._gradient_tensor()
         return [Grid2D(g[:, :, i], self._gridsize, self._origin) for i in range(2)]
 
 
 class TraveltimeGrid3D(BaseGrid3D, BaseTraveltime):
-    """3D traveltime grid, fteikpy/_grid.py:310-427."""
+    """3D traveltime grid, fteikpy/_grid.py:161-278."""
 
     def __init__(self, grid, gridsize, origin, source, gradient, vzero, sign=None, sets_done=None):
         super().__init__(grid=grid, gridsize=gridsize, origin=np.asarray(origin, dtype=np.float64),
@@ -390,15 +432,15 @@ class TraveltimeGrid3D(BaseGrid3D, BaseTraveltime):
                          sets_done=sets_done)
 
     def __call__(self, points, fill_value=np.nan):
-        """Trilinear apparent velocity interpolation (fteikpy/_grid.py:342-368)."""
+        """Trilinear apparent velocity interpolation (fteikpy/_grid.py:193-219)."""
         return self._vinterp(points, fill_value)
 
     def raytrace(self, points, stepsize=None, max_step=None, honor_grid=False):
-        """3D a posteriori ray-tracing (fteikpy/_grid.py:370-414)."""
+        """3D a posteriori ray-tracing (fteikpy/_grid.py:221-265)."""
         return self._raytrace(points, stepsize, max_step, honor_grid)
 
     @property
     def gradient(self):
-        """Z, X and Y gradient grids as a list of Grid3D (fteikpy/_grid.py:416-427)."""
+        """Z, X and Y gradient grids as a list of Grid3D (fteikpy/_grid.py:267-278)."""
         g = self._gradient_tensor()
         return [Grid3D(g[:, :, :, i], self._gridsize, self._origin) for i in range(3)]

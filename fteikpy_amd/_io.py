@@ -13,57 +13,53 @@ from ._grid import TraveltimeGrid2D, TraveltimeGrid3D
 from ._solver import Eikonal2D, Eikonal3D
 
 
+_GRIDS = (Eikonal2D, Eikonal3D, TraveltimeGrid2D, TraveltimeGrid3D)
+_NODE_GRIDS = (TraveltimeGrid2D, TraveltimeGrid3D)
+
+
+def _numbered(label, k):
+    """Data array names: the first of a kind keeps the bare label, later ones are numbered
+    from 2 ("Velocity", "Velocity 2", ...)."""
+    return label if k == 1 else f"{label} {k}"
+
+
+def _elevation_frame(zxy):
+    """Columns in the grids' (z, x[, y]) axis order -> the mesh frame (x, y, -z): depth
+    becomes elevation and a 2D grid lies in the y = 0 plane."""
+    zxy = np.asarray(zxy, dtype=np.float64)
+    y = zxy[:, 2] if zxy.shape[1] == 3 else np.zeros(len(zxy))
+    return np.column_stack((zxy[:, 1], y, -zxy[:, 0]))
+
+
 def grid_to_meshio(*args):
     """Return a meshio.Mesh from Eikonal2D/3D and TraveltimeGrid2D/3D objects
-    (fteikpy/_io.py:7-106)."""
+    (fteikpy/_io.py:7-106).  The first argument sets the mesh: its cells are the velocity
+    cells (a traveltime grid's nodes are their corners).  Velocity models become cell data,
+    traveltimes and their gradient direction point data."""
     import meshio
 
     for i, arg in enumerate(args):
-        if not isinstance(arg, (Eikonal2D, Eikonal3D, TraveltimeGrid2D, TraveltimeGrid3D)):
+        if not isinstance(arg, _GRIDS):
             raise ValueError(f"argument {i + 1} is not a supported grid")
-        if i == 0:
-            ndim = arg._ndim
-            shape = arg.shape
-            gridsize = arg.gridsize
-            origin = arg.origin
+    first = args[0]
+    ndim = first._ndim
+    ncell = [n - 1 for n in first.shape] if isinstance(first, _NODE_GRIDS) else list(first.shape)
+    zxy, cells = _structured_mesh(ncell, first.gridsize, first.origin)
+    points = _elevation_frame(zxy)
 
-    node_grid = isinstance(args[0], (TraveltimeGrid2D, TraveltimeGrid3D))
-    if ndim == 2:
-        nz, nx = shape
-        dz, dx = gridsize
-        z0, x0 = origin
-        if node_grid:
-            nz, nx = nz - 1, nx - 1
-        points, cells = _mesh_2d(nx, nz, dx, dz, x0, z0)
-        points = np.column_stack((points, np.zeros(len(points))))[:, [0, 2, 1]]
-    else:
-        nz, nx, ny = shape
-        dz, dx, dy = gridsize
-        z0, x0, y0 = origin
-        if node_grid:
-            nz, nx, ny = nz - 1, nx - 1, ny - 1
-        points, cells = _mesh_3d(nx, ny, nz, dx, dy, dz, x0, y0, z0)
-    points[:, 2] *= -1.0  # depth -> elevation
-
+    counts = {}
     point_data, cell_data = {}, {}
-    vel_count = tt_count = 0
     for arg in args:
-        if isinstance(arg, (Eikonal2D, Eikonal3D)):
-            vel_count += 1
-            name = f"Velocity {vel_count}" if vel_count > 1 else "Velocity"
-            cell_data[name] = [_ravel_grid(arg.grid, ndim)]
-        else:
-            tt_count += 1
-            name = f"Traveltime {tt_count}" if tt_count > 1 else "Traveltime"
-            point_data[name] = _ravel_grid(arg.grid, ndim)
-            if arg._sign is not None or arg._gradient_dev is not None:
-                name = f"Gradient {tt_count}" if tt_count > 1 else "Gradient"
-                g = np.column_stack([_ravel_grid(c.grid, ndim) for c in arg.gradient])
-                if ndim == 2:
-                    g = np.column_stack((g, np.zeros(len(points))))
-                g = g[:, [1, 2, 0]]
-                g[:, 2] *= -1.0
-                point_data[name] = g
+        kind = "Traveltime" if isinstance(arg, _NODE_GRIDS) else "Velocity"
+        counts[kind] = counts.get(kind, 0) + 1
+        k = counts[kind]
+        if kind == "Velocity":
+            cell_data[_numbered(kind, k)] = [_ravel_grid(arg.grid, ndim)]
+            continue
+        point_data[_numbered(kind, k)] = _ravel_grid(arg.grid, ndim)
+        if arg._sign is not None or arg._gradient_dev is not None:
+            comps = np.column_stack([_ravel_grid(c.grid, ndim) for c in arg.gradient])
+            point_data[_numbered("Gradient", k)] = _elevation_frame(comps)
     return meshio.Mesh(points, cells, point_data, cell_data)
 
 
@@ -91,41 +87,33 @@ def ray_to_meshio(*args):
     return meshio.Mesh(points, cells)
 
 
-def _mesh_2d(nx, ny, dx, dy, x0, y0):
-    """Structured quads (fteikpy/_io.py:156-195): points in Fortran order over (x, y),
-    vertices (i, j) (i+1, j) (i+1, j+1) (i, j+1) of each cell, cells in Fortran order."""
-    xs = np.arange(nx + 1) * dx + x0
-    ys = np.arange(ny + 1) * dy + y0
-    X, Y = np.meshgrid(xs, ys, indexing="ij")
-    points = np.column_stack((X.ravel("F"), Y.ravel("F"))).astype(float)
-    I, J = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
-    i, j = I.ravel("F"), J.ravel("F")
+def _structured_mesh(ncell, gridsize, origin):
+    """Nodes and cells of a structured grid of `ncell` cells per (z, x[, y]) axis
+    (fteikpy/_io.py:156-237).
 
-    def idx(a, b):
-        return a + b * (nx + 1)
-
-    cells = np.column_stack((idx(i, j), idx(i + 1, j), idx(i + 1, j + 1), idx(i, j + 1)))
-    return points, [("quad", cells)]
-
-
-def _mesh_3d(nx, ny, nz, dx, dy, dz, x0, y0, z0):
-    """Structured hexahedra (fteikpy/_io.py:198-237): points in C order over (x, y, z),
-    vertices of the lower face then the upper face of each cell, cells in C order."""
-    xs = np.arange(nx + 1) * dx + x0
-    ys = np.arange(ny + 1) * dy + y0
-    zs = np.arange(nz + 1) * dz + z0
-    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
-    points = np.column_stack((X.ravel(), Y.ravel(), Z.ravel())).astype(float)
-    I, J, K = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
-    i, j, k = I.ravel(), J.ravel(), K.ravel()
-
-    def idx(a, b, c):
-        return (a * (ny + 1) + b) * (nz + 1) + c
-
-    cells = np.column_stack((idx(i, j, k), idx(i + 1, j, k), idx(i + 1, j + 1, k),
-                             idx(i, j + 1, k), idx(i, j, k + 1), idx(i + 1, j, k + 1),
-                             idx(i + 1, j + 1, k + 1), idx(i, j + 1, k + 1)))
-    return points, [("hexahedron", cells)]
+    Returns the node coordinates as (z, x[, y]) columns and the cell blocks.  Nodes and
+    cells are numbered in the export order of the data arrays (`_ravel_grid`): in 2D C
+    order over (z, x), in 3D C order over (x, y, z).  A quad lists its corners
+    (x, z), (x+1, z), (x+1, z+1), (x, z+1); a hexahedron its lower face
+    (x, y, z), (x+1, y, z), (x+1, y+1, z), (x, y+1, z), then the same four at z+1."""
+    nd = len(ncell)
+    nodes = [n + 1 for n in ncell]
+    axes = [o + d * np.arange(n) for o, d, n in zip(origin, gridsize, nodes)]
+    mesh = np.meshgrid(*axes, indexing="ij")
+    zxy = np.column_stack([_ravel_grid(m, nd) for m in mesh])
+    # node number of (z, x[, y]) index arrays, and the corner offsets of a cell
+    if nd == 2:
+        number = lambda i: i[0] * nodes[1] + i[1]  # noqa: E731
+        corners = [(0, 0), (0, 1), (1, 1), (1, 0)]
+        ctype = "quad"
+    else:
+        number = lambda i: (i[1] * nodes[2] + i[2]) * nodes[0] + i[0]  # noqa: E731
+        corners = [(0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1),
+                   (1, 0, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1)]
+        ctype = "hexahedron"
+    low = [_ravel_grid(m, nd) for m in np.meshgrid(*[np.arange(n) for n in ncell], indexing="ij")]
+    conn = np.column_stack([number([a + o for a, o in zip(low, off)]) for off in corners])
+    return zxy, [(ctype, conn)]
 
 
 def _ravel_grid(grid, ndim):

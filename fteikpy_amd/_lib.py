@@ -98,7 +98,7 @@ SIGNATURES = {
 
 def build():
     """Compile libfteik_amd.so for gfx950 in-tree."""
-    subprocess.run(["make", "-s", "-C", os.path.join(_HERE, "csrc")], check=True)
+    subprocess.run(["make", "-s", "-j8", "-C", os.path.join(_HERE, "csrc")], check=True)
 
 
 def lib():

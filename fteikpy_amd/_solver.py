@@ -26,12 +26,17 @@ def _nsweep(nsweep):
 
 
 class _SlownessCache:
-    """slowness = 1.0 / velocity computed once on the device (fteikpy/_solver.py:49)."""
+    """slowness = 1.0 / velocity on the device (fteikpy/_solver.py:49), recomputed when the
+    velocity tensor is replaced or written in place (torch's version counter of the tensor,
+    shared with its views, counts in-place writes; uploading a handed-out .grid is one)."""
 
     def _slowness(self):
-        if getattr(self, "_slow_dev", None) is None or self._slow_src is not self._dev:
-            self._slow_dev = (1.0 / self._dev).contiguous()
-            self._slow_src = self._dev
+        vel = self._device()
+        key = (vel, vel._version)
+        cached = getattr(self, "_slow_key", None)
+        if cached is None or cached[0] is not vel or cached[1] != key[1]:
+            self._slow_dev = (1.0 / vel).contiguous()
+            self._slow_key = key
         return self._slow_dev
 
 
@@ -107,8 +112,15 @@ class Eikonal3D(BaseGrid3D, _SlownessCache):
         n = len(rel)
         nz, nx, ny = self.shape
         dev = device()
-        tt = out if out is not None else torch.empty((n, nz + 1, nx + 1, ny + 1),
-                                                     dtype=torch.float64, device=dev)
+        shape = (n, nz + 1, nx + 1, ny + 1)
+        if out is not None:
+            if (tuple(out.shape) != shape or out.dtype != torch.float64
+                    or out.device.type != "cuda" or not out.is_contiguous()):
+                raise ValueError(f"out must be a contiguous float64 device tensor of shape "
+                                 f"{shape}")
+            tt = out
+        else:
+            tt = torch.empty(shape, dtype=torch.float64, device=dev)
         sgn = torch.empty((n, nz + 1, nx + 1, ny + 1), dtype=torch.uint8,
                           device=dev) if return_gradient else None
         vzero = torch.empty(n, dtype=torch.float64, device=dev)

@@ -109,18 +109,25 @@ def test_constant_gradient_error(fa):
     assert rel.max() < 0.05 and rel.mean() < 0.02
 
 
-def test_marmousi_like_workflow(fa):
-    """The reference's demo pipeline (.github/sample.py) on a layered, faulted model of the
-    Marmousi model's size: smooth(0.05) -> solve((0, 0), nsweep=3, gradients) -> 100 rays
+def test_marmousi_workflow(fa):
+    """The reference's demo pipeline (.github/sample.py:18-32) on its own Marmousi model
+    (tests/golden/marmousi.npy, 300 x 1000 float32 m/s, loaded without pickle): velocities
+    in km/s on a 10 m grid, smooth(0.05) -> solve((0, 0), nsweep=3, gradients) -> 100 rays
     from the surface -> traveltimes along the rays, each step against the oracle."""
+    import os
+
     from oracle import gridops
 
-    v, d = synthetic.marmousi_like()
-    eik = fa.Eikonal2D(v, d)
+    vel = np.load(os.path.join(os.path.dirname(__file__), "golden", "marmousi.npy"),
+                  allow_pickle=False)
+    assert vel.shape == (300, 1000) and vel.dtype == np.float32
+    v = np.asarray(vel * 1.0e-3, dtype=np.float64)
+    d = (0.01, 0.01)
+    eik = fa.Eikonal2D(vel * 1.0e-3, gridsize=d)
     eik.smooth(0.05)
     vs = gridops.smooth(v, d, 0.05)
     assert np.array_equal(eik.grid, vs)
-    print("marmousi-like: solve", flush=True)
+    print("marmousi: solve", flush=True)
     tt = eik.solve((0.0, 0.0), nsweep=3, return_gradient=True)
     ref, rgrad, rvz, _ = oracle.solve2d(1.0 / vs, *d, np.zeros(2), nsweep=3, grad=True)
     assert np.array_equal(tt.grid, ref)
@@ -129,7 +136,7 @@ def test_marmousi_like_workflow(fa):
     rays = tt.raytrace(ends)
     z, x = [np.arange(n + 1) * di for n, di in zip(v.shape, d)]
     gz, gx = rgrad[..., 0], rgrad[..., 1]
-    # the reference's default max_step (fteikpy/_grid.py:242-251): node counts x spacing
+    # the reference's default max_step (fteikpy/_grid.py:129-133): node counts x spacing
     nzn, nxn = len(z), len(x)
     max_step = int(2.0 * ((nzn * d[0]) ** 2 + (nxn * d[1]) ** 2) ** 0.5 / min(d))
     ref_rays = oracle.ray2d(z, x, gz, gx, ends, np.zeros(2), min(d), max_step)
@@ -138,3 +145,5 @@ def test_marmousi_like_workflow(fa):
         np.testing.assert_allclose(r, rr, rtol=1e-10, atol=1e-10)
         np.testing.assert_allclose(tt(r), oracle.vinterp2d(z, x, ref, rr, np.zeros(2), rvz),
                                    rtol=1e-10, atol=1e-12)
+    # the demo's last figure: the largest traveltime of the grid (seconds)
+    assert 2.0 < tt.grid.max() < 4.0

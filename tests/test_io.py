@@ -37,18 +37,31 @@ def _loop_mesh_3d(nx, ny, nz, dx, dy, dz, x0, y0, z0):
     return np.array(pts, dtype=float), np.array(cells)
 
 
+def _reference_frame_2d(pts):
+    # the reference's 2D steps (fteikpy/_io.py:50-57, :71): append y = 0, swap the last two
+    # columns, negate the vertical one
+    p = np.column_stack((pts, np.zeros(len(pts))))[:, [0, 2, 1]]
+    p[:, 2] *= -1.0
+    return p
+
+
 @pytest.mark.parametrize("n", [(2, 3), (4, 1), (5, 7)])
 def test_mesh_2d_matches_per_cell_construction(n):
-    pts, cells = _io._mesh_2d(*n, 0.5, 2.0, -1.0, 3.0)
-    rp, rc = _loop_mesh_2d(*n, 0.5, 2.0, -1.0, 3.0)
-    assert np.array_equal(pts, rp) and cells[0][0] == "quad" and np.array_equal(cells[0][1], rc)
+    nx, nz = n
+    zxy, cells = _io._structured_mesh((nz, nx), (2.0, 0.5), (3.0, -1.0))
+    rp, rc = _loop_mesh_2d(nx, nz, 0.5, 2.0, -1.0, 3.0)
+    assert np.array_equal(_io._elevation_frame(zxy), _reference_frame_2d(rp))
+    assert cells[0][0] == "quad" and np.array_equal(cells[0][1], rc)
 
 
 @pytest.mark.parametrize("n", [(2, 3, 4), (1, 1, 1), (3, 5, 2)])
 def test_mesh_3d_matches_per_cell_construction(n):
-    pts, cells = _io._mesh_3d(*n, 0.5, 2.0, 1.5, -1.0, 3.0, 7.0)
-    rp, rc = _loop_mesh_3d(*n, 0.5, 2.0, 1.5, -1.0, 3.0, 7.0)
-    assert np.array_equal(pts, rp) and cells[0][0] == "hexahedron"
+    nx, ny, nz = n
+    zxy, cells = _io._structured_mesh((nz, nx, ny), (1.5, 0.5, 2.0), (7.0, -1.0, 3.0))
+    rp, rc = _loop_mesh_3d(nx, ny, nz, 0.5, 2.0, 1.5, -1.0, 3.0, 7.0)
+    rp = rp.copy()
+    rp[:, 2] *= -1.0
+    assert np.array_equal(_io._elevation_frame(zxy), rp) and cells[0][0] == "hexahedron"
     assert np.array_equal(cells[0][1], rc)
 
 
