@@ -297,7 +297,6 @@ class BaseTraveltime:
         replace the times it belongs to."""
         if self._gradient_dev is None and self._sign is not None:
             self._gradient_tensor()
-        self._sign = None
 
     def _gradient_tensor(self):
         if self._gradient_dev is None:
