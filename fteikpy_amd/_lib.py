@@ -52,6 +52,10 @@ SIGNATURES = {
     "fteik3d_solve": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _dp, _c_int,
                                _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
                                _vp, ctypes.c_size_t, _c_int, _vp]),
+    "fteik3d_scratch_capacity": (ctypes.c_size_t, []),
+    "fteik3d_solve_noalloc": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _dp,
+                                       _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp,
+                                       _vp, _vp, _vp]),
     "fteik3d_gradient": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _dp,
                                   _c_dbl, _vp, _vp]),
     "fteik3d_history_size": (ctypes.c_size_t, [ctypes.c_longlong, _c_int]),

@@ -41,6 +41,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <mutex>
 #include <cmath>
 #include <vector>
 
@@ -1262,6 +1263,61 @@ extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double 
     return solve3d_impl(slow, nz, nx, ny, dz, dx, dy, src, nsrc, nsweep, max_sets, want_grad,
                         tile_a, group, schedule, tt, sgn, vzero, sets_done, workspace,
                         workspace_bytes, use_graph, stream, nullptr, 0, nullptr);
+}
+
+// ---------------------------------------------------------------------------------------
+// Solve without allocation: the workspace is a file-scope device array
+// ---------------------------------------------------------------------------------------
+
+namespace {
+
+// Each device that runs the library holds one copy of this array until the process ends.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` per device.  Host code uses these addresses, never the
+// array's name: on the host the name is a stand-in, not device memory.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(void **out) {
+    int dev = 0;
+    FTEIK_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return FTEIK_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        FTEIK_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return FTEIK_ERR_WORKSPACE;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" size_t fteik3d_scratch_capacity(void) { return kScratchBytes; }
+
+extern "C" int fteik3d_solve_noalloc(const double *slow, int nz, int nx, int ny, double dz,
+                                     double dx, double dy, const double *src, int nsrc,
+                                     int nsweep, int max_sets, int want_grad, int tile_a,
+                                     int group, double *tt, uint8_t *sgn, double *vzero,
+                                     int *sets_done, void *stream) {
+    // argument and capacity checks come before any HIP call
+    if (nz < 1 || nx < 1 || ny < 1 || nsrc < 1 || nsrc > 65535 || !slow || !src || !tt ||
+        !vzero || !sets_done || (want_grad && !sgn))
+        return FTEIK_ERR_ARG;
+    if (fteik3d_workspace_size(nz, nx, ny, nsrc) > kScratchBytes) return FTEIK_ERR_WORKSPACE;
+    void *ws = nullptr;
+    if (int st = scratch_address(&ws)) return st;
+    // diagonal schedule launched directly (a captured graph would be instantiated, which
+    // allocates); every byte of the workspace the kernels read is written by the solve's
+    // own fill, clear and seed kernels first
+    return solve3d_impl(slow, nz, nx, ny, dz, dx, dy, src, nsrc, nsweep, max_sets, want_grad,
+                        tile_a, group, 1, tt, sgn, vzero, sets_done, ws, kScratchBytes, 0,
+                        stream, nullptr, 0, nullptr);
 }
 
 extern "C" size_t fteik3d_history_size(long long events, int max_passes) {

@@ -82,9 +82,30 @@ int fteik3d_solve(const double *slow, int nz, int nx, int ny, double dz, double 
                   int *sets_done, void *workspace, size_t workspace_bytes, int use_graph,
                   void *stream);
 
+/* Bytes of the file-scope device scratch array of fteik3d_solve_noalloc (64 MiB). */
+size_t fteik3d_scratch_capacity(void);
+
+/*
+ * No counterpart in the reference: fteik3d_solve for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The call allocates nothing: its workspace is a file-scope device array of
+ * fteik3d_scratch_capacity() bytes that each device running the library holds until the
+ * process ends (its address is looked up once per device on the first call there).
+ * Arguments and results as fteik3d_solve with schedule 1 and use_graph 0; problems whose
+ * fteik3d_workspace_size exceeds the capacity return FTEIK_ERR_WORKSPACE before anything
+ * is enqueued.  All calls on one device share the array: calls that overlap on one device
+ * (concurrent host threads, or different streams) race and must be serialized by the
+ * caller.  Like fteik3d_solve, it reads `src` from host memory and synchronizes the stream
+ * before its first kernel (and once per set when converging).
+ */
+int fteik3d_solve_noalloc(const double *slow, int nz, int nx, int ny, double dz, double dx,
+                          double dy, const double *src, int nsrc, int nsweep, int max_sets,
+                          int want_grad, int tile_a, int group, double *tt, uint8_t *sgn,
+                          double *vzero, int *sets_done, void *stream);
+
 /*
  * Replaces the gradient post-pass of fteik3d (fteikpy/_fteik/_fteik3d.py:464-488) that
- * fills TraveltimeGrid3D.gradient (fteikpy/_grid.py:416-427).
+ * fills TraveltimeGrid3D.gradient (fteikpy/_grid.py:267-278).
  * tt, sgn: device, one source ([nz+1, nx+1, ny+1]); vzero: that source's value;
  * src: host [3] relative to the origin.  ttgrad: device [nz+1, nx+1, ny+1, 3] (output).
  */
@@ -124,7 +145,7 @@ size_t fteik3d_adjoint_history_workspace_size(int nz, int nx, int ny, int nsrc,
  * No counterpart in the reference: the exact gradient of a receiver-traveltime loss L with
  * respect to slowness -- reverse mode through the sweep history of fteik3d_solve_logged --
  * for T_rec[s, r] = vinterp3d of the solved grids (TraveltimeGrid3D.__call__,
- * fteikpy/_grid.py:342-368, on Eikonal3D.solve grids, fteikpy/_solver.py:120-126).
+ * fteikpy/_grid.py:193-219, on Eikonal3D.solve grids, fteikpy/_solver.py:120-126).
  * Passes are reversed last to first; the winning candidate of every logged change is
  * recomputed on the grid as it stood after its pass.
  *
@@ -149,7 +170,7 @@ size_t fteik3d_adjoint_workspace_size(int nz, int nx, int ny, int nsrc);
 /*
  * No counterpart in the reference: the gradient of a receiver-traveltime loss L with
  * respect to slowness, for T_rec[s, r] = vinterp3d of converged fteik3d grids (what
- * TraveltimeGrid3D.__call__ returns, fteikpy/_grid.py:342-368, on the grids of
+ * TraveltimeGrid3D.__call__ returns, fteikpy/_grid.py:193-219, on the grids of
  * Eikonal3D.solve, fteikpy/_solver.py:120-126).  The upwind stencil of every node is
  * recomputed from the converged times and the adjoint system is solved exactly in reverse
  * dependency order.
@@ -296,7 +317,7 @@ int fteik3d_interp(const double *v, int n0, int n1, int n2, int stride, const do
 /*
  * Apparent-velocity trilinear traveltime interpolation, replaces vinterp3d
  * (fteikpy/_interp/_vinterp3d.py:7-291), called from TraveltimeGrid3D.__call__
- * (fteikpy/_grid.py:342-368).  tt: device [n0, n1, n2]; src: host [3] ABSOLUTE source
+ * (fteikpy/_grid.py:193-219).  tt: device [n0, n1, n2]; src: host [3] ABSOLUTE source
  * coordinates; vzero: slowness of the source cell.
  */
 int fteik3d_vinterp(const double *tt, int n0, int n1, int n2, const double *origin,
@@ -318,7 +339,7 @@ int fteik2d_vinterp(const double *tt, int n0, int n1, const double *origin, cons
 
 /*
  * A-posteriori ray tracing, replaces ray3d (fteikpy/_fteik/_ray3d.py:9-208), called from
- * TraveltimeGrid3D.raytrace (fteikpy/_grid.py:370-414).
+ * TraveltimeGrid3D.raytrace (fteikpy/_grid.py:221-265).
  * grad: device [n0, n1, n2, 3] normalized gradient (fteik3d_gradient output).
  * p: device [n, 3] end points (receivers), src: host [3] ABSOLUTE source.
  * rays: device [n, max_step, 3] work buffer in receiver-to-source order (as the

@@ -57,6 +57,32 @@ def test_host_argument_checks():
         _lib.check(_lib.ERR_MAX_STEP)
 
 
+def test_noalloc_capacity_checks_before_any_device_work():
+    """fteik3d_solve_noalloc refuses problems beyond its fixed scratch array, and bad
+    pointers, before any HIP call."""
+    from fteikpy_amd import _lib
+
+    L = _lib.lib()
+    cap = L.fteik3d_scratch_capacity()
+    assert cap == 64 << 20
+    src = np.zeros(3)
+    sp = src.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: both calls return before any use
+    n = 1
+    while L.fteik3d_workspace_size(99, 99, 99, n) <= cap:
+        n += 1
+    assert L.fteik3d_workspace_size(99, 99, 99, n - 1) <= cap
+    st = L.fteik3d_solve_noalloc(fake, 99, 99, 99, 1.0, 1.0, 1.0, sp, n, 2, 10, 0, 0, 0, fake,
+                                 None, fake, fake, None)
+    assert st == _lib.ERR_WORKSPACE
+    st = L.fteik3d_solve_noalloc(None, 9, 9, 9, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, 0, fake,
+                                 None, fake, fake, None)
+    assert st == _lib.ERR_ARG
+    st = L.fteik3d_solve_noalloc(fake, 9, 9, 9, 1.0, 1.0, 1.0, sp, 1, 2, 10, 1, 0, 0, fake,
+                                 None, fake, fake, None)
+    assert st == _lib.ERR_ARG  # signs requested without a sign buffer
+
+
 def test_source_check_before_any_device_work():
     """An out-of-bound source is rejected from the host (the reference raises before
     allocating, fteikpy/_fteik/_fteik3d.py:402-407): fake non-null device pointers are
