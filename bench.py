@@ -24,6 +24,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# LDS: 128 bytes per clock per CU, 256 CUs, 2.4 GHz peak engine clock
+LDS_PEAK_GBS = 128 * 256 * 2.4
 BYTES_PER_NSU = 24  # FP64 tt read + tt write + slowness read per node update
 
 # HBM bytes per launch of the dominant kernel, sweep3d_diag_kernel<false, false, 2>, on the
@@ -33,7 +35,11 @@ BYTES_PER_NSU = 24  # FP64 tt read + tt write + slowness read per node update
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Commands and figures:
 # profiles/sweep3d_traffic__inclusions_160x320x320_256src.md
 PMC_TRAFFIC = {"grid": (160, 320, 320), "sources": 256, "tile_a": 0, "group": 2,
-               "fetch_bytes": 159025.98 * 2.0 * 1024, "write_bytes": 8467.68 * 1.0 * 1024}
+               "fetch_bytes": 159025.98 * 2.0 * 1024, "write_bytes": 8467.68 * 1.0 * 1024,
+               # SQ_INSTS_LDS per launch x 64 lanes x 5.4 B, the mean bytes per lane of the
+               # step loop's LDS instructions (8 ds_read2_b64, 2 ds_read_b64, 6 ds_write_b64, 18
+               # ds_read_u8, 6 ds_write_b8 per two unrolled steps)
+               "lds_bytes": 2.14981e6 * 64 * 5.4}
 
 
 def main():
@@ -59,19 +65,47 @@ def main():
                     help="skip the error report against closed-form traveltimes")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the uneven multi-model batch (48 models, 16 2D + 32 3D)")
+    ap.add_argument("--profile-c2", action="store_true",
+                    help="only run the single-source 200^3 solve (a warm-up and 3 converged "
+                         "solves) and print its times, for profiling")
+    ap.add_argument("--profile-adjoint", action="store_true",
+                    help="only run the gradient workload (one warm-up and one timed "
+                         "forward+adjoint iteration), for profiling")
     ap.add_argument("--profile-2d", action="store_true",
                     help="only run the 2D workloads (the plain 2D case at nsweep 2, the 2D "
                          "models of the uneven batch) and print their times, for profiling")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only copy GIB GiB once on the device (torch clone) and exit: the "
                          "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--row-calibration", action="store_true",
+                    help="only copy 72-byte rows (9 doubles at element offset 7 of every "
+                         "321-double row of a 161 x 321 x 321 grid, the sweep kernel's staging "
+                         "reads) 8 times and exit: the row-granular reference for FETCH_SIZE")
     args = ap.parse_args()
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
         return
+    if args.row_calibration:
+        row_calibration()
+        return
 
     if args.profile_2d:
         profile_2d()
+        return
+    if args.profile_c2 or args.profile_adjoint:
+        import torch
+
+        import fteikpy_amd
+        from fteikpy_amd import synthetic
+
+        dev = torch.device("cuda", 0)
+        with torch.cuda.stream(torch.cuda.Stream(device=dev)):
+            if args.profile_c2:
+                out = single_source_c2(torch, fteikpy_amd, synthetic, dev, with_cpu=False)
+            else:
+                args.adj_steps, args.no_tune = 1, True
+                out = fwd_adjoint(args, torch, fteikpy_amd, synthetic, dev)
+        print(json.dumps(out), flush=True)
         return
 
     if args.gpus != 1:
@@ -155,7 +189,8 @@ def main():
         adj = (fwd_adjoint(args, torch, fteikpy_amd, synthetic, dev) if args.adj_steps > 0
                else None)
         ana = None if args.no_analytic else analytic_errors(torch, fteikpy_amd, dev)
-        c2 = None if args.no_analytic else single_source_c2(torch, fteikpy_amd, synthetic, dev)
+        c2 = None if args.no_analytic else single_source_c2(torch, fteikpy_amd, synthetic, dev,
+                                                            with_cpu=not args.no_cpu)
         ub = None if args.no_batch else uneven_batch(torch, fteikpy_amd, synthetic, dev)
         p2d = None if args.no_analytic else plain_2d(torch, fteikpy_amd, synthetic, dev,
                                                      with_cpu=not args.no_cpu)
@@ -182,25 +217,27 @@ def main():
     if not args.no_cpu:
         import oracle
 
-        ncores = min(os.cpu_count() or 1, 16)
+        ncores, host = host_cpu()
         ncs = args.cpu_sources or ncores
         nsets = args.cpu_sets
         tc = time.perf_counter()
         oracle.solve3d(1.0 / v, *d, src[:ncs], nsweep=nsets, nthreads=ncores)
         tcpu = time.perf_counter() - tc
-        log(f"cpu baseline: {tcpu:.1f} s")
+        log(f"cpu baseline: {tcpu:.1f} s on {ncores} threads ({host['model']})")
         # the CPU path evaluates every node in every set, so a converged grid costs its set
         # count (measured above for the same sources) times the per-set time
         mean_sets = float(k_sets[:ncs].mean())
         cpu = {"value": ncs * nsets / tcpu / mean_sets, "unit": "source-grids/s",
-               "cores": ncores, "kind": "port",
+               "cores": ncores, "kind": "port", "host": host,
                "sample": f"{ncs} of the {nsrc} sources x {nsets} sweep sets in {tcpu:.1f} s on "
                          f"{ncores} threads, scaled to the {mean_sets:.1f} sets those sources "
                          "need to converge; oracle C restatement of the reference's numba "
-                         "path, OpenMP over sources"}
+                         "path (its k,j,i loop nesting), OpenMP over sources like the "
+                         "reference's prange"}
 
     torch.cuda.empty_cache()
     copy_gbs = stream_copy(torch, dev, stream)
+    lds_b = PMC_TRAFFIC["lds_bytes"] if traffic else None
     us_launch = gpu_s / launches * 1e6
     res = {
         "metric": "converged FP64 source-grids/sec 3D, batched vs plain GPU; fwd+adjoint s/iter",
@@ -232,16 +269,24 @@ def main():
                      "algorithmic_bytes_per_launch": alg_bytes / launches,
                      "launches": launches, "us_per_launch": us_launch,
                      "traffic_GBps": traffic / us_launch / 1e3 if traffic else None,
+                     "frac_traffic": traffic / us_launch / 1e3 / HBM_PEAK_GBS if traffic else None,
+                     "lds_bytes_per_launch_est": lds_b,
+                     "lds_GBps_est": lds_b / us_launch / 1e3 if lds_b else None,
+                     "lds_peak_GBps": LDS_PEAK_GBS,
                      "stream_copy_GBps": copy_gbs, "achieved_over_stream_copy": achieved / copy_gbs,
-                     "note": "achieved = algorithmic bytes per launch (24 B per node update x 8 "
-                             "passes x nodes x sets executed, over the launches of the timed "
-                             "solves) / mean launch time (HIP events on the kernels' stream); "
-                             "traffic = measured HBM bytes per launch (rocprofv3 FETCH_SIZE + "
-                             "WRITE_SIZE, copy-calibrated; null for other configurations): "
-                             "below the algorithmic bytes because inactive bricks and "
-                             "unchanged nodes are skipped; traffic_GBps = traffic / mean launch "
-                             "time; stream_copy_GBps = a 1 GiB device copy (read + write bytes) "
-                             "timed here"},
+                     "note": "achieved (frac) = WORK-EQUIVALENT bandwidth: algorithmic bytes "
+                             "per launch (24 B per node update x 8 passes x nodes x sets "
+                             "executed, over the launches of the timed solves, counting the "
+                             "node updates the kernel proves unnecessary and skips) / mean "
+                             "launch time (HIP events on the kernels' stream); traffic = "
+                             "measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                             "calibrated on copies of known size; null for other "
+                             "configurations) and frac_traffic = traffic / mean launch time / "
+                             "peak, the bandwidth actually used; lds_bytes_per_launch_est = "
+                             "LDS instructions per launch (rocprofv3 SQ_INSTS_LDS) x 64 lanes x "
+                             "the mean bytes per lane of the kernel's LDS instruction mix "
+                             "(an upper bound: inactive lanes counted); stream_copy_GBps = a "
+                             "1 GiB device copy (read + write bytes) timed here"},
         "cpu_baseline": cpu,
         "fwd_adjoint": adj,
         "analytic": ana,
@@ -255,6 +300,29 @@ def main():
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_cpu():
+    """(threads used, description) of the host CPU: every logical CPU this process may run
+    on (its affinity mask), capped by OMP_NUM_THREADS when the environment sets a share for
+    this process, and the CPU model from /proc/cpuinfo."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS")
+    used = min(avail, int(share)) if share and share.isdigit() and int(share) > 0 else avail
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return used, {"model": model, "logical_cpus_available": avail,
+                  "omp_num_threads": share, "threads_used": used}
 
 
 def copy_calibration(gib):
@@ -271,6 +339,25 @@ def copy_calibration(gib):
     dt = time.perf_counter() - t0
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "seconds": dt,
                       "checksum": float(y[::4096].sum())}), flush=True)
+
+
+def row_calibration(reps=8, shape=(161, 321, 321), first=7):
+    """Copies of 72-byte rows: every (z, x) row of a traveltime-sized grid contributes
+    elements first..first+8 (the sweep kernel stages 9 x 9 node blocks whose rows start one
+    element before a multiple of 8), each copy one kernel reading z*x*72 bytes in 72-byte
+    pieces and writing them contiguously."""
+    import torch
+
+    g = torch.ones(shape, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    out = None
+    for _ in range(reps):
+        out = g[:, :, first:first + 9].clone()
+    torch.cuda.synchronize()
+    rows = shape[0] * shape[1]
+    print(json.dumps({"row_copy_dispatches": reps, "bytes_read_per_dispatch": rows * 72,
+                      "bytes_written_per_dispatch": rows * 72,
+                      "checksum": float(out.sum())}), flush=True)
 
 
 def headline_post(torch, fa, eik, src, rec, d, nrays=64):
@@ -301,10 +388,14 @@ def headline_post(torch, fa, eik, src, rec, d, nrays=64):
                     "to the host as the reference's lists of arrays)"}
 
 
-def single_source_c2(torch, fa, synthetic, dev, reps=3):
+def single_source_c2(torch, fa, synthetic, dev, reps=3, with_cpu=True):
     """The single-source configuration: 200^3 layered model with a smooth perturbation,
     5 m, one off-grid source, converged solve (tt stays on the device).  Best of `reps`
-    after a warm-up, with its algorithmic rate (24 B per node update)."""
+    after a warm-up, with its algorithmic rate (24 B per node update).  Beside it the
+    oracle (the reference's numba path restated in C, its loop nesting) on one host thread
+    (one source: the reference's prange over sources has nothing to spread): one sweep set
+    timed, scaled to the set count of the converged solve (the CPU path evaluates every
+    node in every set)."""
     v, d, src, rec = synthetic.model_3d_layers()
     eik = fa.Eikonal3D(v, d)
     eik.solve_tensors(src, nsweep=None)
@@ -330,7 +421,21 @@ def single_source_c2(torch, fa, synthetic, dev, reps=3):
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         full = t if full is None else min(full, t)
+    cpu = None
+    if with_cpu:
+        import oracle
+
+        tc = time.perf_counter()
+        oracle.solve3d(1.0 / v, *d, src, nsweep=1, nthreads=1)
+        t1 = time.perf_counter() - tc
+        _, host = host_cpu()
+        cpu = {"value": 1.0 / (t1 * sets), "unit": "source-grids/s", "cores": 1,
+               "kind": "port", "host_model": host["model"],
+               "sample": f"one sweep set of the source in {t1:.1f} s on one thread, scaled to "
+                         f"the {sets} sets of the converged solve"}
+        log(f"single source: cpu {t1:.1f} s per set")
     return {"grid": list(v.shape), "spacing_m": d[0], "sets": sets, "seconds": best,
+            "cpu_baseline": cpu,
             "source_grids_per_s": 1.0 / best, "achieved_GBps": alg / best / 1e9,
             "seconds_with_gradient_and_10000_receivers": full,
             "note": "wall time of one converged solve (seeding, sets, convergence checks), "
@@ -405,7 +510,7 @@ def plain_2d(torch, fa, synthetic, dev, with_cpu=True, reps=3):
     if with_cpu:
         import oracle
 
-        ncores = min(os.cpu_count() or 1, 16)
+        ncores, _ = host_cpu()
         tc = time.perf_counter()
         oracle.solve2d(1.0 / v, *d, srcs, nsweep=0, nthreads=ncores)
         entry["cpu_oracle_s"] = time.perf_counter() - tc

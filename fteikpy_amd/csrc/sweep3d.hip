@@ -38,6 +38,7 @@
 #include "fteik_device.hpp"
 #include "node_update.hpp"
 #include "history.hpp"
+#include "schedule.hpp"
 
 #include <algorithm>
 #include <climits>
@@ -48,8 +49,8 @@
 namespace fteik {
 namespace {
 
-constexpr int kLanesB = 8;      // brick width along x (oriented b)
-constexpr int kLanesC = 8;      // brick width along y (oriented c)
+constexpr int kLanesB = sched::kLanesB;  // brick width along x (oriented b)
+constexpr int kLanesC = sched::kLanesC;  // brick width along y (oriented c)
 constexpr int kSkew = kLanesB + kLanesC - 2;  // steps between first and last lane = 14
 constexpr int kPlane = (kLanesB + 1) * (kLanesC + 1);  // 81 entries: node or cell block
 constexpr int kPrefetch = 1;    // planes resident in LDS ahead of lane (0,0)
@@ -104,32 +105,10 @@ struct SweepArgs : Stencil3 {
     long long log_cap;
 };
 
-// pass order of the reference (_fteik3d.py:217-391): z flips at pass 4, y at 2, x at 1
-__host__ __device__ __forceinline__ int pass_sz(int p) { return (p & 4) ? -1 : 1; }
-__host__ __device__ __forceinline__ int pass_sx(int p) { return (p & 1) ? -1 : 1; }
-__host__ __device__ __forceinline__ int pass_sy(int p) { return (p & 2) ? -1 : 1; }
-
-// brick (BA, BB, BC) number t of anti-diagonal d, enumerated with BA outermost
-__device__ __forceinline__ void decode_brick(int d, int t, int nba, int nbb, int nbc, int *BA,
-                                             int *BB, int *BC) {
-    const int amin = max(0, d - (nbb - 1) - (nbc - 1));
-    const int amax = min(nba - 1, d);
-    *BA = amin;
-    *BB = 0;
-    *BC = 0;
-    for (int ba = amin; ba <= amax; ++ba) {
-        const int e = d - ba;
-        const int bmin = max(0, e - (nbc - 1)), bmax = min(nbb - 1, e);
-        const int cnt = bmax - bmin + 1;
-        if (t < cnt) {
-            *BA = ba;
-            *BB = bmin + t;
-            *BC = e - *BB;
-            return;
-        }
-        t -= cnt;
-    }
-}
+using sched::decode_brick;
+using sched::pass_sx;
+using sched::pass_sy;
+using sched::pass_sz;
 
 // One brick for W sources (wave w handles list[g0 + w]).  Every early return is uniform
 // across the workgroup.
@@ -549,7 +528,7 @@ __global__ void log_mark_kernel(HistHeader *hdr, unsigned long long *seg,
 // every workgroup still exits.
 template <bool GRAD, int W>
 __global__ __launch_bounds__(64 * W) void sweep3d_persistent_kernel(const SweepArgs A) {
-    __shared__ int s_task;
+    __shared__ int s_task, s_abort;
     const int nact = *A.nlist;
     const int G = (nact + W - 1) / W;
     const long long per_pass = (long long)A.ntiles * G;
@@ -561,47 +540,19 @@ __global__ __launch_bounds__(64 * W) void sweep3d_persistent_kernel(const SweepA
         const long long t = s_task;
         __syncthreads();  // s_task is rewritten only after every thread has read it
         if (t >= ntasks) break;
-        const int p = (int)(t / per_pass);
-        const long long r = t - p * per_pass;
-        // anti-diagonal d: cum_nb[d] * G <= r < cum_nb[d+1] * G
-        const int rq = (int)(r / G);
-        int lo = 0, hi = A.ndiag - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (A.cum_nb[mid] <= rq) lo = mid;
-            else hi = mid - 1;
-        }
-        const int d = lo;
-        const long long j = r - (long long)A.cum_nb[d] * G;
-        const int i = (int)(j / G), g = (int)(j - (long long)i * G);
+        const sched::Task k = sched::decode_task(t, G, A.ntiles, A.cum_nb, A.ndiag);
+        const int p = k.p, g = k.g;
         int BA, BB, BC;
-        decode_brick(d, i, A.nba, A.nbb, A.nbc, &BA, &BB, &BC);
-        const int sz = pass_sz(p), sx = pass_sx(p), sy = pass_sy(p);
-        const int ti = sz > 0 ? BA : A.nba - 1 - BA;
-        const int tj = sx > 0 ? BB : A.nbb - 1 - BB;
-        const int tk = sy > 0 ? BC : A.nbc - 1 - BC;
+        decode_brick(k.d, k.i, A.nba, A.nbb, A.nbc, &BA, &BB, &BC);
+        int ti, tj, tk;
+        sched::brick_tile(p, BA, BB, BC, A.nba, A.nbb, A.nbc, &ti, &tj, &tk);
         int *prog = A.progress + (long long)g * A.ntiles;
 
-        // wait: lanes 0..26 check the 3x3x3 neighbourhood for pass p-1 done, lanes 27..29
-        // the upwind face tiles for pass p done
+        // wait: lane s checks dependency slot s (the 3x3x3 neighbourhood for pass p-1 done,
+        // the upwind face tiles for pass p done)
         if (tid < 64) {
-            int need = 0, idx = -1;
-            if (tid < 27) {
-                const int ii = ti + tid % 3 - 1, jj = tj + (tid / 3) % 3 - 1, kk = tk + tid / 9 - 1;
-                if (p > 0 && ii >= 0 && ii < A.nba && jj >= 0 && jj < A.nbb && kk >= 0 &&
-                    kk < A.nbc) {
-                    need = p;
-                    idx = (ii * A.nbb + jj) * A.nbc + kk;
-                }
-            } else if (tid < 30) {
-                const int ax = tid - 27;
-                const int ii = ti - (ax == 0 ? sz : 0), jj = tj - (ax == 1 ? sx : 0),
-                          kk = tk - (ax == 2 ? sy : 0);
-                if (ii >= 0 && ii < A.nba && jj >= 0 && jj < A.nbb && kk >= 0 && kk < A.nbc) {
-                    need = p + 1;
-                    idx = (ii * A.nbb + jj) * A.nbc + kk;
-                }
-            }
+            int need = 0;
+            const int idx = sched::task_dependency(tid, p, ti, tj, tk, A.nba, A.nbb, A.nbc, &need);
             bool ok = idx < 0;
             const unsigned long long t_start = wall_clock64();
             while (!__all(ok)) {
@@ -609,17 +560,25 @@ __global__ __launch_bounds__(64 * W) void sweep3d_persistent_kernel(const SweepA
                     ok = __hip_atomic_load(prog + idx, __ATOMIC_ACQUIRE,
                                            __HIP_MEMORY_SCOPE_AGENT) >= need;
                 if (__all(ok)) break;
+                // a wait that gave up anywhere ends every wait of the launch: the result is
+                // invalid already, and the launch drains without further time-outs
+                if (__hip_atomic_load(A.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+                    break;
                 if (wall_clock64() - t_start > A.wait_ticks) {
                     if (tid == 0) atomicExch(A.error, 1);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(2);
             }
+            if (tid == 0)
+                s_abort = __hip_atomic_load(A.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
-        __threadfence();  // acquire for every wave: see the producers' data
-        brick<GRAD, false, W>(A, p, BA, BB, BC, g * W, nact);
-        __threadfence();  // release this thread's writes
+        if (!s_abort) {
+            __threadfence();  // acquire for every wave: see the producers' data
+            brick<GRAD, false, W>(A, p, BA, BB, BC, g * W, nact);
+            __threadfence();  // release this thread's writes
+        }
         __syncthreads();
         if (tid == 0)
             __hip_atomic_store(prog + (ti * A.nbb + tj) * A.nbc + tk, p + 1, __ATOMIC_RELEASE,
@@ -822,16 +781,7 @@ bool place_source3(const double *s, int nz, int nx, int ny, double dz, double dx
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
-int bricks_on_diag(int d, int nba, int nbb, int nbc) {
-    int total = 0;
-    const int amin = std::max(0, d - (nbb - 1) - (nbc - 1)), amax = std::min(nba - 1, d);
-    for (int ba = amin; ba <= amax; ++ba) {
-        const int e = d - ba;
-        const int bmin = std::max(0, e - (nbc - 1)), bmax = std::min(nbb - 1, e);
-        if (bmax >= bmin) total += bmax - bmin + 1;
-    }
-    return total;
-}
+using sched::bricks_on_diag;
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
@@ -904,14 +854,8 @@ struct Ws3 {
     uint64_t *faces;
 };
 
-// Upper bounds over every tile_a >= 1: tiles of the memory grid and anti-diagonals.
-size_t max_tiles(int nz, int nx, int ny) {
-    return (size_t)(nz + 1) * ((nx + 1 + kLanesB - 1) / kLanesB) *
-           ((ny + 1 + kLanesC - 1) / kLanesC);
-}
-size_t max_diag(int nz, int nx, int ny) {
-    return (size_t)(nz + 1) + (nx + 1 + kLanesB - 1) / kLanesB + (ny + 1 + kLanesC - 1) / kLanesC;
-}
+using sched::max_diag;
+using sched::max_tiles;
 
 // Face bytes are kept for bricks of at least kFaceMinTile planes (tiles of any smaller
 // height use the whole-tile bytes of their neighbours).
