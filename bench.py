@@ -71,6 +71,15 @@ def main():
     ap.add_argument("--profile-adjoint", action="store_true",
                     help="only run the gradient workload (one warm-up and one timed "
                          "forward+adjoint iteration), for profiling")
+    ap.add_argument("--profile-batch", action="store_true",
+                    help="only run the uneven multi-model batch (warm-up and one timed call of "
+                         "all 48 models), for profiling")
+    ap.add_argument("--profile-plain", action="store_true",
+                    help="only run the per-source loop of the batched workload (a warm-up and "
+                         "--plain converged single-source solves), for profiling")
+    ap.add_argument("--profile-post", action="store_true",
+                    help="only run the batched workload's solve with gradient signs and its "
+                         "per-source gradient grids, receivers and rays, for profiling")
     ap.add_argument("--profile-2d", action="store_true",
                     help="only run the 2D workloads (the plain 2D case at nsweep 2, the 2D "
                          "models of the uneven batch) and print their times, for profiling")
@@ -91,6 +100,9 @@ def main():
 
     if args.profile_2d:
         profile_2d()
+        return
+    if args.profile_batch or args.profile_plain or args.profile_post:
+        profile_sections(args)
         return
     if args.profile_c2 or args.profile_adjoint:
         import torch
@@ -442,6 +454,42 @@ def single_source_c2(torch, fa, synthetic, dev, reps=3, with_cpu=True):
                     "brick height and group at the library defaults for one source; then "
                     "of the same solve with gradient signs, the gradient grids and the "
                     "traveltimes at 10,000 receivers"}
+
+
+def profile_sections(args):
+    """One bench section alone, on one stream, for a profiler: the uneven batch (a warm-up
+    and one timed call), the per-source loop of the batched workload (a warm-up and
+    args.plain solves), or the batched workload's solve with signs and its per-source
+    gradients, receivers and rays.  Prints the section's JSON."""
+    import torch
+
+    import fteikpy_amd as fa
+    from fteikpy_amd import synthetic
+
+    dev = torch.device("cuda", 0)
+    with torch.cuda.stream(torch.cuda.Stream(device=dev)):
+        if args.profile_batch:
+            out = uneven_batch(torch, fa, synthetic, dev, reps=1)
+        else:
+            shape = tuple(int(x) for x in args.shape.split(","))
+            v, d, src, rec = synthetic.model_3d_inclusions(shape=shape)
+            eik = fa.Eikonal3D(v, d)
+            if args.profile_post:
+                out = headline_post(torch, fa, eik, src, rec, d)
+            else:
+                n = max(1, args.plain)
+                eik.solve_tensors(src[:1], nsweep=None)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                sets = []
+                for i in range(n):
+                    _, _, _, k = eik.solve_tensors(src[i:i + 1], nsweep=None)
+                    sets.append(int(k[0]))
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                out = {"plain_sources": n, "seconds": dt, "source_grids_per_s": n / dt,
+                       "sets": sets}
+    print(json.dumps(out), flush=True)
 
 
 def profile_2d(reps=2):
