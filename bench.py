@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--plain", type=int, default=4, help="sources timed in the plain loop")
     ap.add_argument("--plain-tile", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0, help="batched brick height (0 = default)")
+    ap.add_argument("--group", type=int, default=0,
+                    help="batched sources per workgroup (0 = the autotuner's choice or default)")
     ap.add_argument("--shape", type=str, default="160,320,320")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
@@ -143,7 +145,7 @@ def main():
 
     tune = None
     with torch.cuda.stream(stream):
-        if not args.no_tune and not args.tile:
+        if not args.no_tune and not args.tile and not args.group:
             # brick height and group for this workload, timed on it before the warm-up
             tune = fteikpy_amd.autotune.tune3d(eik, src, tiles=(16, 32, 0),
                                                use_graph=use_graph)
@@ -152,7 +154,8 @@ def main():
                 f"({tune['seconds']:.1f} s)")
         out = torch.empty((nsrc, nz + 1, nx + 1, ny + 1), dtype=torch.float64, device=dev)
         for _ in range(args.warmup):
-            eik.solve_tensors(src, nsweep=None, tile_a=args.tile, use_graph=use_graph, out=out)
+            eik.solve_tensors(src, nsweep=None, tile_a=args.tile, group=args.group,
+                              use_graph=use_graph, out=out)
         torch.cuda.synchronize()
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
@@ -161,7 +164,7 @@ def main():
         ev0.record(stream)
         for _ in range(args.steps):
             tt, _, vz, sets = eik.solve_tensors(src, nsweep=None, tile_a=args.tile,
-                                                use_graph=use_graph, out=out)
+                                                group=args.group, use_graph=use_graph, out=out)
             sets_total += int(sets.sum())
         ev1.record(stream)
         torch.cuda.synchronize()
@@ -215,7 +218,7 @@ def main():
     # kernel per (pass, brick anti-diagonal), until its last source converged; brick
     # height 0 is the library default (autotune.default_tile)
     tile = args.tile or (tune["tile_a"] if tune else 0)
-    group = tune["group"] if tune else (2 if nsrc >= 8 else 1)
+    group = args.group or (tune["group"] if tune else (2 if nsrc >= 8 else 1))
     t = tile if tile > 0 else fteikpy_amd.autotune.default_tile(nsrc, nz + 1)
     t = min(t, nz + 1)
     ndiag = -(-(nz + 1) // t) + -(-(nx + 1) // 8) + -(-(ny + 1) // 8) - 2
@@ -270,7 +273,7 @@ def main():
                    "sets_per_source": [int(k_sets.min()), int(k_sets.max())],
                    "hipgraph": use_graph,
                    "tile_a": args.tile or (tune["tile_a"] if tune else 0),
-                   "group": tune["group"] if tune else 0},
+                   "group": args.group or (tune["group"] if tune else 0)},
         "autotune": tune,
         "plain_value": nplain / plain_s if plain_s else None,
         "plain_sources_timed": nplain,
