@@ -93,6 +93,7 @@ struct SweepArgs : Stencil3 {
     long long node_stride;  // nodes per source grid
     int nz, nx, ny;         // cells
     int NZ, NX, NY;         // nodes
+    int nsrc;               // sources of the model (list, sets_done entries)
     int tile_a;             // z extent of a tile / brick
     int nba, nbb, nbc;      // tile counts along z, x, y (memory grid, nodes 0..N-1)
     int ntiles, ndiag;
@@ -122,8 +123,28 @@ __device__ __forceinline__ void log_pad(const SweepArgs &A, unsigned long long p
         if (e < (unsigned long long)A.log_cap) A.log_node[e] = kLogUnused;
 }
 
+// Skip test of a brick for source s0: slot q = 0 reads the tile byte, q = 1..7 what the 7
+// upwind neighbour tiles (bit 0: along z, 1: x, 2: y) hold next to it: the face toward
+// this tile (its z face for the neighbours below or above, its x face for the x and xy
+// neighbours, a superset of the edge or corner they share), or their whole-tile byte
+// without face bytes.  Zero when the neighbour lies outside the grid.
+__device__ __forceinline__ unsigned neighbour_byte(const SweepArgs &A, int s0, int q, int ti, int tj,
+                                                   int tk, int sz, int sx, int sy) {
+    const int ii = ti - ((q & 1) ? sz : 0);
+    const int jj = tj - ((q & 2) ? sx : 0);
+    const int kk = tk - ((q & 4) ? sy : 0);
+    if (ii < 0 || ii >= A.nba || jj < 0 || jj >= A.nbb || kk < 0 || kk >= A.nbc) return 0u;
+    const long long t = s0 * (long long)A.ntiles + ((long long)ii * A.nbb + jj) * A.nbc + kk;
+    if (q == 0 || A.faces == nullptr) return A.tsum[t];
+    const int f = (q & 1) ? (sz > 0 ? 1 : 0) : ((q & 2) ? (sx > 0 ? 3 : 2) : (sy > 0 ? 5 : 4));
+    return (unsigned)(A.faces[t] >> (8 * f)) & 0xffu;
+}
+
+// pre < 0: the brick tests each wave's source for skipping itself; else bit w of pre says
+// whether wave w's source is active (the caller tested it, the same test).
 template <bool GRAD, bool LOG, int W>
-__device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int g0, int nact) {
+__device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int g0, int nact,
+                      int pre = -1) {
     constexpr int kThreads = 64 * W;
     constexpr int kT = kPlane * W;                           // node entries (W blocks)
     constexpr int NT = (kT + kThreads - 1) / kThreads;       // = 2
@@ -149,33 +170,49 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
     const long long tiles = A.ntiles;
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int bb = lane >> 3, cc = lane & 7;
+    const int b = b0 + bb, c = c0 + cc;
+
+    // oriented -> memory index maps
+    auto node_i = [&](int a) { return sz > 0 ? a : A.NZ - 1 - a; };
+    auto node_j = [&](int x) { return sx > 0 ? x : NX - 1 - x; };
+    auto node_k = [&](int y) { return sy > 0 ? y : NY - 1 - y; };
+    auto cell_i = [&](int a) { return sz > 0 ? a : A.nz - 1 - a; };
+    auto cell_j = [&](int x) { return sx > 0 ? x : A.nx - 1 - x; };
+    auto cell_k = [&](int y) { return sy > 0 ? y : A.ny - 1 - y; };
+    const long long plane_nodes = (long long)NX * NY;
+    const long long plane_cells = (long long)A.nx * A.ny;
+
+    // The brick's global loads come in two rounds, each issued before any of it is waited
+    // on.  Round 1: the source of each of the W blocks, the first-set test and the guard.
+    // Every source runs the first set, so "not the first set" is read from any source's
+    // set count (that of source g0) rather than behind the list entry.
+    // nact < 0: the caller left the count of sources still changing to this round (the
+    // list entries are read ahead of it, within the list's nsrc entries).
+    int srcw[W];  // source of block w, -1 for none
+#pragma unroll
+    for (int w = 0; w < W; ++w) srcw[w] = g0 + w < A.nsrc ? A.list[g0 + w] : -1;
+    const bool first = A.sets_done[g0] == 0;  // g0 < nsrc for every launched block
+    const bool guard_clear = *A.guard == 0;
+    if (nact < 0) nact = *A.nlist;
+    if (g0 >= nact) return;  // no source left in this group (uniform)
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if (g0 + w >= nact) srcw[w] = -1;
+    const int s0 = srcw[wave];
+    const bool has = s0 >= 0;
+    // skipping is allowed from the second set on, and in the first set while the guard holds
+    const bool may_skip = has && (!first || guard_clear);
+
+    // Round 2: the skip bytes of this wave's source (lanes 0..7).  A wave with a source
+    // sweeps unless it may skip and neither the tile nor an upwind neighbour holds a change
+    // next to it (pre >= 0: the caller decided).
     {
-        const bool has = g0 + wave < nact;
-        const int s0 = has ? A.list[g0 + wave] : 0;
+        const unsigned skip_byte =
+            may_skip && lane < 8 ? neighbour_byte(A, s0, lane, ti, tj, tk, sz, sx, sy) : 0u;
         bool on = has;
-        if (has && (A.sets_done[s0] > 0 || *A.guard == 0)) {
-            // lane 0 reads the tile byte, lanes 1..7 what its 7 upwind neighbour tiles hold
-            // next to it: the face toward this tile (its z face for the neighbours below or
-            // above, its x face for the x and xy neighbours, a superset of the edge or
-            // corner they share), or their whole-tile byte without face bytes
-            unsigned any = 0;
-            if (lane < 8) {
-                const int ii = ti - ((lane & 1) ? sz : 0);
-                const int jj = tj - ((lane & 2) ? sx : 0);
-                const int kk = tk - ((lane & 4) ? sy : 0);
-                if (ii >= 0 && ii < A.nba && jj >= 0 && jj < A.nbb && kk >= 0 && kk < A.nbc) {
-                    const long long t = s0 * tiles + ((long long)ii * A.nbb + jj) * A.nbc + kk;
-                    if (lane == 0 || A.faces == nullptr) {
-                        any = A.tsum[t];
-                    } else {
-                        const int f = (lane & 1) ? (sz > 0 ? 1 : 0)
-                                                 : ((lane & 2) ? (sx > 0 ? 3 : 2) : (sy > 0 ? 5 : 4));
-                        any = (unsigned)(A.faces[t] >> (8 * f)) & 0xffu;
-                    }
-                }
-            }
-            on = __any(any != 0u);
-        }
+        if (may_skip) on = __any(skip_byte != 0u);
+        if (pre >= 0) on = ((pre >> wave) & 1) != 0;
         if (lane == 0) wave_on[wave] = on ? 1 : 0;
     }
     __syncthreads();
@@ -185,29 +222,19 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
     if (!block_on) return;
 
     const bool has_src = wave_on[wave] != 0;
-    const int src = has_src ? A.list[g0 + wave] : 0;
-    const int bb = lane >> 3, cc = lane & 7;
-    const int b = b0 + bb, c = c0 + cc;
+    const int src = has_src ? s0 : 0;
     // ing: this lane's column is in the grid; inb: its nodes are updated in this pass (the
     // oriented planes b = 0 and c = 0 are not)
     const bool ing = has_src && b >= 0 && b <= NX - 1 && c >= 0 && c <= NY - 1;
     const bool inb = ing && b >= 1 && c >= 1;
-
-    // oriented -> memory index maps
-    auto node_i = [&](int a) { return sz > 0 ? a : A.NZ - 1 - a; };
-    auto node_j = [&](int x) { return sx > 0 ? x : NX - 1 - x; };
-    auto node_k = [&](int y) { return sy > 0 ? y : NY - 1 - y; };
-    auto cell_i = [&](int a) { return sz > 0 ? a : A.nz - 1 - a; };
-    auto cell_j = [&](int x) { return sx > 0 ? x : A.nx - 1 - x; };
-    auto cell_k = [&](int y) { return sy > 0 ? y : A.ny - 1 - y; };
-
     double *tt = A.tt + (long long)src * A.node_stride;
     uint8_t *sg = GRAD ? A.sgn + (long long)src * A.node_stride : nullptr;
-    const long long plane_nodes = (long long)NX * NY;
-    const long long plane_cells = (long long)A.nx * A.ny;
 
-    // Staging: node entries e = tid + 64 W n (< 81 W; node (eb, ec) of wave e / 81's
-    // block), traveltime and change byte.  Nodes outside the grid read as Big.
+    // Round 3, active bricks only, every load issued before any is waited on: the first
+    // planes for the ring, the planes in flight in registers, this lane's slowness cells.
+    // Staging: node entries e = tid + 64 W n (< 81 W; node (eb, ec) of block e / 81),
+    // traveltime and change byte, of the blocks whose source sweeps; nodes outside the
+    // grid read as Big.
     long long t_off[NT];
     bool t_use[NT], t_ok[NT];
 #pragma unroll
@@ -221,7 +248,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
             const int eb = b0 - 1 + r / 9, ec = c0 - 1 + r % 9;
             if (wave_on[w] && eb >= 0 && eb <= NX - 1 && ec >= 0 && ec <= NY - 1) {
                 t_ok[n] = true;
-                t_off[n] = (long long)A.list[g0 + w] * A.node_stride + (long long)node_j(eb) * NY +
+                t_off[n] = (long long)srcw[w] * A.node_stride + (long long)node_j(eb) * NY +
                            node_k(ec);
             }
         }
@@ -256,25 +283,53 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
                 cslot[tid + kThreads * n] = (uint8_t)v.c[n];
             }
     };
-
-    for (int p = -1; p < min(kPrefetch, na); ++p) {
-        Stage v;
-        load_plane(p, v);
-        store_plane(p, v);
-    }
-    // planes kPrefetch .. kPrefetch+kDepth-1 in flight in registers
+    // planes -1 .. kPrefetch-1 go to the ring first, planes kPrefetch .. kPrefetch+kDepth-1
+    // stay in flight in registers
+    Stage v0[kPrefetch + 1];
+#pragma unroll
+    for (int p = -1; p < kPrefetch; ++p)
+        if (p < na) load_plane(p, v0[p + 1]);
     Stage rv[kDepth];
 #pragma unroll
     for (int k = 0; k < kDepth; ++k)
         if (kPrefetch + k < na) load_plane(kPrefetch + k, rv[k]);
+
+    // Slowness: each lane streams the 4 cells (b-1 | b') x (c-1 | c') around its column (' =
+    // clamped to the grid's last cell) on the cell plane a' = min(a, nz - 1) of its current
+    // node from global memory, one step ahead; the previous step's 4 are the upwind plane's
+    // (a - 1 = the previous a').  Read-only data shared by every source, it stays in cache,
+    // and keeping it out of the LDS ring lets 3 workgroups' waves share a SIMD.  Offsets
+    // within a cell plane fit 32 bits (checked on the host).
+    int scol[4];
+    {
+        const int bm = min(max(b - 1, 0), A.nx - 1), bp = min(max(b, 0), A.nx - 1);
+        const int cm = min(max(c - 1, 0), A.ny - 1), cp = min(max(c, 0), A.ny - 1);
+        scol[0] = cell_j(bm) * A.ny + cell_k(cm);
+        scol[1] = cell_j(bm) * A.ny + cell_k(cp);
+        scol[2] = cell_j(bp) * A.ny + cell_k(cm);
+        scol[3] = cell_j(bp) * A.ny + cell_k(cp);
+    }
+    auto load_cells = [&](int p, double (&v)[4]) {
+        if (inb) {
+            const int pc = min(max(p, -1), na - 1);
+            const long long pb = (long long)cell_i(min(a_first + pc, A.nz - 1)) * plane_cells;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = A.slow[pb + scol[q]];
+        }
+    };
+    double sp[4] = {0.0, 0.0, 0.0, 0.0}, sc[4] = {0.0, 0.0, 0.0, 0.0};
+    load_cells(-bb - cc - 1, sp);
+    load_cells(-bb - cc, sc);
+
+#pragma unroll
+    for (int p = -1; p < kPrefetch; ++p)
+        if (p < na) store_plane(p, v0[p + 1]);
     __syncthreads();
     // From the second set on, a node none of whose 7 upwind neighbours changed during the
     // last 8 passes sees exactly the candidates of its previous update in this direction,
     // while its own time can only have decreased since: it cannot change and is skipped.
     // In the first set the bytes only accumulate, and the same test finds 7 neighbours at
     // their initial time (valid when the guard holds).
-    const bool first = !has_src || A.sets_done[src] == 0;
-    const bool may_skip = has_src && (!first || *A.guard == 0);
     const unsigned pbit = 1u << pass;
     const unsigned keep = first ? 0xffu : ~pbit;  // bits kept besides this pass's
     uint8_t *cg = A.chg + (long long)src * A.node_stride;
@@ -307,33 +362,6 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
         if (mi == tile_ilo) face_zlo |= c0b;
         if (mi == tile_ihi) face_zhi |= c0b;
     }
-    // Slowness: each lane streams the 4 cells (b-1 | b') x (c-1 | c') around its column (' =
-    // clamped to the grid's last cell) on the cell plane a' = min(a, nz - 1) of its current
-    // node from global memory, one step ahead; the previous step's 4 are the upwind plane's
-    // (a - 1 = the previous a').  Read-only data shared by every source, it stays in cache,
-    // and keeping it out of the LDS ring lets 3 workgroups' waves share a SIMD.  Offsets
-    // within a cell plane fit 32 bits (checked on the host).
-    int scol[4];
-    {
-        const int bm = min(max(b - 1, 0), A.nx - 1), bp = min(max(b, 0), A.nx - 1);
-        const int cm = min(max(c - 1, 0), A.ny - 1), cp = min(max(c, 0), A.ny - 1);
-        scol[0] = cell_j(bm) * A.ny + cell_k(cm);
-        scol[1] = cell_j(bm) * A.ny + cell_k(cp);
-        scol[2] = cell_j(bp) * A.ny + cell_k(cm);
-        scol[3] = cell_j(bp) * A.ny + cell_k(cp);
-    }
-    auto load_cells = [&](int p, double (&v)[4]) {
-        if (inb) {
-            const int pc = min(max(p, -1), na - 1);
-            const long long pb = (long long)cell_i(min(a_first + pc, A.nz - 1)) * plane_cells;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) v[q] = A.slow[pb + scol[q]];
-        }
-    };
-    double sp[4] = {0.0, 0.0, 0.0, 0.0}, sc[4] = {0.0, 0.0, 0.0, 0.0};
-    load_cells(-bb - cc - 1, sp);
-    load_cells(-bb - cc, sc);
-
     const int nsteps = na + kSkew;
     // ring slots advance by one per step: this lane's current plane p = s - bb - cc sits in
     // slot (p+1) mod kRing, the plane written back at step s in slot (s-kSkew+1) mod kRing
@@ -480,12 +508,10 @@ template <bool GRAD, bool LOG, int W>
 __global__ __launch_bounds__(64 * W)
 __attribute__((amdgpu_waves_per_eu((LOG && W <= 4) ? 3 : 1, (LOG && W <= 4) ? 3 : 10)))
 void sweep3d_diag_kernel(const SweepArgs A) {
-    const int nact = *A.nlist;
-    const int g0 = blockIdx.y * W;
-    if (g0 >= nact) return;
     int BA, BB, BC;
     decode_brick(A.diag, blockIdx.x, A.nba, A.nbb, A.nbc, &BA, &BB, &BC);
-    brick<GRAD, LOG, W>(A, A.pass, BA, BB, BC, g0, nact);
+    // the count of sources still changing is read with the brick's first loads
+    brick<GRAD, LOG, W>(A, A.pass, BA, BB, BC, blockIdx.y * W, -1);
 }
 
 // Batch of models: one launch per (pass, anti-diagonal d) covers the bricks of d of every
@@ -496,20 +522,27 @@ __global__ __launch_bounds__(64 * W) void sweep3d_multi_kernel(const SweepArgs *
                                                                const int *cum, int nmodel,
                                                                int pass, int diag) {
     const int b = blockIdx.x;
-    int lo = 0, hi = nmodel - 1;  // the last model whose first block is <= b
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (cum[mid] <= b) lo = mid;
-        else hi = mid - 1;
+    int lo;  // the last model whose first block is <= b (cum is non-decreasing, cum[0] = 0)
+    if (nmodel <= 64) {
+        // one load per lane and a vote instead of a search's dependent loads
+        const int i = threadIdx.x & 63;
+        const bool le = i < nmodel && cum[i] <= b;
+        lo = (int)__popcll(__ballot(le)) - 1;
+    } else {
+        int l = 0, hi = nmodel - 1;
+        while (l < hi) {
+            const int mid = (l + hi + 1) >> 1;
+            if (cum[mid] <= b) l = mid;
+            else hi = mid - 1;
+        }
+        lo = l;
     }
     const SweepArgs A = models[lo];
     const int local = b - cum[lo];
     const int t = local / A.gmax, g = local - t * A.gmax;
-    const int nact = *A.nlist;
-    if (g * W >= nact) return;
     int BA, BB, BC;
     decode_brick(diag, t, A.nba, A.nbb, A.nbc, &BA, &BB, &BC);
-    brick<GRAD, false, W>(A, pass, BA, BB, BC, g * W, nact);
+    brick<GRAD, false, W>(A, pass, BA, BB, BC, g * W, -1);
 }
 
 // Logged solve: closes the current pass's segment of the history.
@@ -980,6 +1013,7 @@ void make_args(const double *slow, int nz, int nx, int ny, double dz, double dx,
     a.node_stride = nn;
     a.nz = nz; a.nx = nx; a.ny = ny;
     a.NZ = NZ; a.NX = NX; a.NY = NY;
+    a.nsrc = nsrc;
     if (tile_a <= 0) tile_a = default_tile(nsrc, NZ);
     a.tile_a = std::min(tile_a, NZ);
     a.nba = ceil_div(NZ, a.tile_a);
