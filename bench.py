@@ -30,16 +30,16 @@ BYTES_PER_NSU = 24  # FP64 tt read + tt write + slowness read per node update
 
 # HBM bytes per launch of the dominant kernel, sweep3d_diag_kernel<false, false, 2>, on the
 # default workload (160 x 320 x 320, 256 sources, the library's default brick height, 27
-# planes here, 2 sources per workgroup): rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script, averaged
-# over the 26832 launches of one converged solve and scaled by a 1 GiB device copy
-# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Commands and figures:
-# profiles/sweep3d_traffic__inclusions_160x320x320_256src.md
+# planes here, 2 sources per workgroup, the overlapped schedule): rocprofv3 --pmc
+# FETCH_SIZE and --pmc WRITE_SIZE runs of this script, averaged over the 19812 launches of
+# one converged solve and scaled by a 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x
+# 1.0).  Commands and figures: profiles/sweep3d_traffic__inclusions_160x320x320_256src.md
 PMC_TRAFFIC = {"grid": (160, 320, 320), "sources": 256, "tile_a": 0, "group": 2,
-               "fetch_bytes": 159025.98 * 2.0 * 1024, "write_bytes": 8467.68 * 1.0 * 1024,
+               "fetch_bytes": 216735.5 * 2.0 * 1024, "write_bytes": 11469.5 * 1.0 * 1024,
                # SQ_INSTS_LDS per launch x 64 lanes x 5.4 B, the mean bytes per lane of the
                # step loop's LDS instructions (8 ds_read2_b64, 2 ds_read_b64, 6 ds_write_b64, 18
                # ds_read_u8, 6 ds_write_b8 per two unrolled steps)
-               "lds_bytes": 2.14981e6 * 64 * 5.4}
+               "lds_bytes": 2.9115596e6 * 64 * 5.4}
 
 
 def main():
@@ -214,15 +214,13 @@ def main():
     alg_bytes = sets_total * 8 * nodes * BYTES_PER_NSU
     achieved = alg_bytes / gpu_s / 1e9
     value = nsrc * args.steps / wall
-    # launches of the sweep kernel in the timed region: every set of a batch launches one
-    # kernel per (pass, brick anti-diagonal), until its last source converged; brick
-    # height 0 is the library default (autotune.default_tile)
+    # launches of the sweep kernel in the timed region: every set of a batch makes the
+    # same launches (the library's count for this grid, brick height and the default
+    # overlapped schedule), until its last source converged
     tile = args.tile or (tune["tile_a"] if tune else 0)
     group = args.group or (tune["group"] if tune else (2 if nsrc >= 8 else 1))
-    t = tile if tile > 0 else fteikpy_amd.autotune.default_tile(nsrc, nz + 1)
-    t = min(t, nz + 1)
-    ndiag = -(-(nz + 1) // t) + -(-(nx + 1) // 8) + -(-(ny + 1) // 8) - 2
-    launches = args.steps * int(k_sets.max()) * 8 * ndiag
+    per_set = fteikpy_amd._lib.lib().fteik3d_sweep_launches(nz, nx, ny, nsrc, tile, 2)
+    launches = args.steps * int(k_sets.max()) * per_set
     traffic = None
     if (tuple(shape) == PMC_TRAFFIC["grid"] and nsrc == PMC_TRAFFIC["sources"]
             and tile == PMC_TRAFFIC["tile_a"] and group == PMC_TRAFFIC["group"]):

@@ -53,6 +53,7 @@ SIGNATURES = {
                                _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
                                _vp, ctypes.c_size_t, _c_int, _vp]),
     "fteik3d_scratch_capacity": (ctypes.c_size_t, []),
+    "fteik3d_sweep_launches": (_c_int, [_c_int] * 6),
     "fteik3d_solve_noalloc": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _dp,
                                        _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp,
                                        _vp, _vp, _vp]),

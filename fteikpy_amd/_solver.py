@@ -8,8 +8,11 @@ for a batch of sources).  Keyword-only extensions:
 - ``group``: sources per workgroup (0 = default; 1, 2, 4, 8);
   with both left at 0, a choice recorded by ``fteikpy_amd.autotune.tune3d`` for this grid
   shape and number of sources is used;
-- ``schedule``: "diagonal" (one launch per brick anti-diagonal and pass; the default) or
-  "persistent" (one launch per sweep set; currently refused);
+- ``schedule``: "overlapped" (the default: each launch runs the bricks, of any pass, whose
+  inputs are final, so a pass starts while the previous one finishes its far side; bricks
+  under 8 planes run pass by pass), "diagonal" (one launch per brick anti-diagonal and
+  pass) or "persistent" (one launch per sweep set; currently refused); the results are
+  identical;
 - ``use_graph``: capture one 3D sweep set as a hipGraph and replay it.
 """
 import ctypes
@@ -19,6 +22,9 @@ import numpy as np
 from . import _lib, autotune
 from ._grid import (BaseGrid2D, BaseGrid3D, TraveltimeGrid2D, TraveltimeGrid3D, _ptr, _torch,
                     device, stream_ptr)
+
+
+_SCHEDULES = {"persistent": 0, "diagonal": 1, "overlapped": 2}
 
 
 def _nsweep(nsweep):
@@ -87,7 +93,7 @@ class Eikonal3D(BaseGrid3D, _SlownessCache):
                          origin=origin if origin is not None else np.zeros(3, dtype=np.float64))
 
     def solve(self, sources, nsweep=2, return_gradient=False, *, max_sets=100, tile_a=0,
-              group=0, schedule="diagonal", use_graph=False):
+              group=0, schedule="overlapped", use_graph=False):
         """Solve the eikonal equation for one source [z, x, y] or a list of sources."""
         tt, sgn, vz, nsets = self.solve_tensors(sources, nsweep, return_gradient,
                                                 max_sets=max_sets, tile_a=tile_a, group=group,
@@ -103,7 +109,7 @@ class Eikonal3D(BaseGrid3D, _SlownessCache):
         return grids[0] if single else grids
 
     def solve_tensors(self, sources, nsweep=2, return_gradient=False, *, max_sets=100, tile_a=0,
-                      group=0, schedule="diagonal", use_graph=False, out=None):
+                      group=0, schedule="overlapped", use_graph=False, out=None):
         """Batched solve returning device tensors: (tt [n, nz+1, nx+1, ny+1], signs or None,
         vzero (numpy [n]), sets executed (numpy [n])).  `out` may pass a preallocated tt."""
         torch = _torch()
@@ -133,7 +139,7 @@ class Eikonal3D(BaseGrid3D, _SlownessCache):
             tile_a, group = autotune.lookup3d(self.shape, n)
         args = (_ptr(slow), nz, nx, ny, *self._gridsize, rp, n, _nsweep(nsweep), int(max_sets),
                 int(bool(return_gradient)), int(tile_a), int(group),
-                {"persistent": 0, "diagonal": 1}[schedule], _ptr(tt),
+                _SCHEDULES[schedule], _ptr(tt),
                 _ptr(sgn) if sgn is not None else None, _ptr(vzero), _ptr(sets), _ptr(ws), wsb)
         cur = torch.cuda.current_stream()
         if use_graph and cur.cuda_stream != 0:

@@ -42,6 +42,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <memory>
 #include <mutex>
 #include <cmath>
 #include <vector>
@@ -98,8 +99,11 @@ struct SweepArgs : Stencil3 {
     int nba, nbb, nbc;      // tile counts along z, x, y (memory grid, nodes 0..N-1)
     int ntiles, ndiag;
     int gmax;               // source groups (workgroups per brick)
-    int pass;               // diagonal schedule: pass of this launch
-    int diag;               // diagonal schedule: brick anti-diagonal of this launch
+    int pass;               // pass-by-pass schedule: pass of this launch
+    int diag;               // pass-by-pass schedule: brick anti-diagonal of this launch;
+                            // overlapped schedule: first entry of this launch in tasks
+    const unsigned long long *tasks;  // overlapped schedule: packed bricks (pass, BA, BB,
+                                      // BC) by launch; null: pass-by-pass
     unsigned *log_node;     // logged solve: changed node (source * nodes + node) ...
     double *log_old;        // ... and its time before the change
     unsigned long long *log_count;
@@ -508,10 +512,13 @@ template <bool GRAD, bool LOG, int W>
 __global__ __launch_bounds__(64 * W)
 __attribute__((amdgpu_waves_per_eu((LOG && W <= 4) ? 3 : 1, (LOG && W <= 4) ? 3 : 10)))
 void sweep3d_diag_kernel(const SweepArgs A) {
-    int BA, BB, BC;
-    decode_brick(A.diag, blockIdx.x, A.nba, A.nbb, A.nbc, &BA, &BB, &BC);
+    int pass = A.pass, BA, BB, BC;
+    if (A.tasks)
+        sched::unpack_task(A.tasks[A.diag + blockIdx.x], &pass, &BA, &BB, &BC);
+    else
+        decode_brick(A.diag, blockIdx.x, A.nba, A.nbb, A.nbc, &BA, &BB, &BC);
     // the count of sources still changing is read with the brick's first loads
-    brick<GRAD, LOG, W>(A, A.pass, BA, BB, BC, blockIdx.y * W, -1);
+    brick<GRAD, LOG, W>(A, pass, BA, BB, BC, blockIdx.y * W, -1);
 }
 
 // Batch of models: one launch per (pass, anti-diagonal d) covers the bricks of d of every
@@ -885,6 +892,7 @@ struct Ws3 {
     int *changed, *active, *list, *nlist, *queue, *error, *guard, *cum_nb, *progress;
     uint8_t *chg, *tsum;
     uint64_t *faces;
+    unsigned long long *tasks;
 };
 
 using sched::max_diag;
@@ -897,6 +905,10 @@ size_t face_tiles(int nz, int nx, int ny) {
     return (size_t)((nz + 1 + kFaceMinTile - 1) / kFaceMinTile) *
            ((nx + 1 + kLanesB - 1) / kLanesB) * ((ny + 1 + kLanesC - 1) / kLanesC);
 }
+
+// The overlapped schedule's brick list (8 bricks per tile) is kept for bricks of at least
+// kOverlapMinTile planes; shorter bricks run pass by pass.
+constexpr int kOverlapMinTile = kFaceMinTile;
 
 size_t ws_bytes(int nz, int nx, int ny, int nsrc, Ws3 *w, char *base) {
     const size_t nodes = (size_t)(nz + 1) * (nx + 1) * (ny + 1);
@@ -917,6 +929,7 @@ size_t ws_bytes(int nz, int nx, int ny, int nsrc, Ws3 *w, char *base) {
     char *chg = take(nodes * nsrc);
     char *tsum = take(tiles * nsrc);
     char *faces = take(sizeof(uint64_t) * face_tiles(nz, nx, ny) * nsrc);
+    char *tasks = take(sizeof(unsigned long long) * 8 * face_tiles(nz, nx, ny));
     if (w) {
         w->prm = (SrcParam *)prm;
         w->changed = (int *)changed;
@@ -931,6 +944,7 @@ size_t ws_bytes(int nz, int nx, int ny, int nsrc, Ws3 *w, char *base) {
         w->chg = (uint8_t *)chg;
         w->tsum = (uint8_t *)tsum;
         w->faces = (uint64_t *)faces;
+        w->tasks = (unsigned long long *)tasks;
     }
     return off;
 }
@@ -1023,6 +1037,7 @@ void make_args(const double *slow, int nz, int nx, int ny, double dz, double dx,
     a.ndiag = a.nba + a.nbb + a.nbc - 2;
     a.pass = 0;
     a.diag = 0;
+    a.tasks = nullptr;
     a.log_node = nullptr;
     a.log_old = nullptr;
     a.log_count = nullptr;
@@ -1047,6 +1062,21 @@ extern "C" size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc) {
 }
 
 namespace {
+
+// Overlapped-schedule brick lists by tile grid, built once per grid shape (the 8 most
+// recent shapes are kept).
+std::mutex g_table_mu;
+std::vector<std::shared_ptr<const sched::OverlapTable>> g_tables;
+std::shared_ptr<const sched::OverlapTable> overlap_table_for(int nba, int nbb, int nbc) {
+    std::lock_guard<std::mutex> lock(g_table_mu);
+    for (const auto &t : g_tables)
+        if (t->nba == nba && t->nbb == nbb && t->nbc == nbc) return t;
+    auto t = std::make_shared<sched::OverlapTable>();
+    if (!sched::overlap_table(nba, nbb, nbc, t.get())) return nullptr;
+    if (g_tables.size() >= 8) g_tables.erase(g_tables.begin());
+    g_tables.push_back(t);
+    return t;
+}
 
 // Logged solve: initialises the history header in device memory.
 __global__ void log_init_kernel(HistHeader *hdr, unsigned long long *seg, int max_pass,
@@ -1078,7 +1108,7 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
     // beat 4 and 8 (a wider workgroup waits at every step for its slowest source)
     if (group == 0) group = nsrc >= 8 ? 2 : 1;
     if (group != 1 && group != 2 && group != 4 && group != 8) return FTEIK_ERR_ARG;
-    if (schedule != 0 && schedule != 1) return FTEIK_ERR_ARG;
+    if (schedule < 0 || schedule > 2) return FTEIK_ERR_ARG;
     // The persistent schedule stalled on its first GPU run (dependency waits on per-tile
     // progress counters were not satisfied) and its cause is not established: it is
     // switched off and refused until it is.
@@ -1107,6 +1137,14 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
     for (int d = 0; d < a.ndiag; ++d) cum[d + 1] = cum[d] + nb[d];
     const long long gmax = a.gmax;
     if (8LL * a.ntiles * gmax >= (1LL << 31)) return FTEIK_ERR_ARG;  // task index is an int
+    // overlapped passes: bricks of at least kOverlapMinTile planes, and no history (whose
+    // segments are closed pass by pass); otherwise one pass at a time
+    std::shared_ptr<const sched::OverlapTable> ov;
+    if (schedule == 2 && !hv && a.tile_a >= kOverlapMinTile) {
+        ov = overlap_table_for(a.nba, a.nbb, a.nbc);
+        if (!ov || ov->task.size() > 8 * face_tiles(nz, nx, ny)) return FTEIK_ERR_ARG;
+        a.tasks = w.tasks;
+    }
     const int pblocks = schedule == 0 ? pers_blocks(want_grad != 0, group) : 0;
     if (schedule == 0 && pblocks <= 0) return FTEIK_ERR_ARG;
     {
@@ -1120,6 +1158,10 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
                                stream));
     FTEIK_CHECK(hipMemcpyAsync(w.cum_nb, cum.data(), sizeof(int) * (a.ndiag + 1),
                                hipMemcpyHostToDevice, stream));
+    if (ov)
+        FTEIK_CHECK(hipMemcpyAsync(w.tasks, ov->task.data(),
+                                   sizeof(unsigned long long) * ov->task.size(),
+                                   hipMemcpyHostToDevice, stream));
     // the host vectors go out of scope on return: finish the copies first
     FTEIK_CHECK(hipStreamSynchronize(stream));
     {
@@ -1158,6 +1200,12 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
                                dim3((unsigned)std::min<long long>((np + 255) / 256, 4096)), dim3(256),
                                0, stream, w.queue, w.progress, np);
             FTEIK_DISPATCH_W(launch_pers_t, want_grad != 0, group, pblocks, stream, a);
+        } else if (ov) {
+            for (int l = 0; l < ov->launches(); ++l) {
+                a.diag = ov->first[l];
+                FTEIK_DISPATCH_W(launch_diag_t, want_grad != 0, group,
+                                 dim3(ov->first[l + 1] - ov->first[l], (unsigned)gmax), stream, a);
+            }
         } else {
             for (int p = 0; p < 8; ++p) {
                 a.pass = p;
@@ -1233,6 +1281,22 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
 
 }  // namespace
 
+extern "C" int fteik3d_sweep_launches(int nz, int nx, int ny, int nsrc, int tile_a,
+                                      int schedule) {
+    if (nz < 1 || nx < 1 || ny < 1 || nsrc < 1 || nsrc > 65535 || tile_a < 0 || schedule < 0 ||
+        schedule > 2)
+        return 0;
+    if (schedule == 0) return 1;
+    const int NZ = nz + 1;
+    const int t = std::min(tile_a > 0 ? tile_a : default_tile(nsrc, NZ), NZ);
+    const int nba = ceil_div(NZ, t), nbb = ceil_div(nx + 1, kLanesB), nbc = ceil_div(ny + 1, kLanesC);
+    if (schedule == 2 && t >= kOverlapMinTile) {
+        const auto ov = overlap_table_for(nba, nbb, nbc);
+        return ov ? ov->launches() : 0;
+    }
+    return 8 * (nba + nbb + nbc - 2);
+}
+
 extern "C" int fteik3d_solve(const double *slow, int nz, int nx, int ny, double dz, double dx,
                              double dy, const double *src, int nsrc, int nsweep, int max_sets,
                              int want_grad, int tile_a, int group, int schedule, double *tt,
@@ -1290,11 +1354,11 @@ extern "C" int fteik3d_solve_noalloc(const double *slow, int nz, int nx, int ny,
     if (fteik3d_workspace_size(nz, nx, ny, nsrc) > kScratchBytes) return FTEIK_ERR_WORKSPACE;
     void *ws = nullptr;
     if (int st = scratch_address(&ws)) return st;
-    // diagonal schedule launched directly (a captured graph would be instantiated, which
+    // overlapped schedule launched directly (a captured graph would be instantiated, which
     // allocates); every byte of the workspace the kernels read is written by the solve's
-    // own fill, clear and seed kernels first
+    // own fill, clear and seed kernels or copied from the host first
     return solve3d_impl(slow, nz, nx, ny, dz, dx, dy, src, nsrc, nsweep, max_sets, want_grad,
-                        tile_a, group, 1, tt, sgn, vzero, sets_done, ws, kScratchBytes, 0,
+                        tile_a, group, 2, tt, sgn, vzero, sets_done, ws, kScratchBytes, 0,
                         stream, nullptr, 0, nullptr);
 }
 

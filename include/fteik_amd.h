@@ -62,9 +62,14 @@ size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc);
  * tile_a    nodes per brick along z (0 = library default: from 32 sources 20..28 planes,
  *           the fewest steps down a column; 16 from 16 sources; else 8)
  * group     sources per workgroup (0 = default; 1, 2, 4, 8)
- * schedule  1: one launch per (pass, brick anti-diagonal), ordered by the stream;
+ * schedule  2: overlapped passes: each launch runs every brick, of any pass, whose
+ *           upwind bricks of its pass and whose tile's readers of the previous pass ran
+ *           in earlier launches (508 launches per set instead of 688 for 160x320x320
+ *           cells in 27-plane bricks); bricks under 8 planes run as schedule 1;
+ *           1: one launch per (pass, brick anti-diagonal), ordered by the stream;
  *           0: persistent (one launch per sweep set, tasks wait on per-tile progress
- *           counters) -- currently refused with FTEIK_ERR_UNSUPPORTED
+ *           counters) -- currently refused with FTEIK_ERR_UNSUPPORTED;
+ *           1 and 2 give identical results
  * tt        device [nsrc, nz+1, nx+1, ny+1] traveltimes (output)
  * sgn       device [nsrc, nz+1, nx+1, ny+1] uint8 packed upwind signs or NULL
  * vzero     device [nsrc] slowness of each source cell (output)
@@ -82,6 +87,11 @@ int fteik3d_solve(const double *slow, int nz, int nx, int ny, double dz, double 
                   int *sets_done, void *workspace, size_t workspace_bytes, int use_graph,
                   void *stream);
 
+/* No counterpart in the reference (instrumentation): launches of the sweep kernel per
+ * sweep set that fteik3d_solve makes for this grid, source count, brick height (0 = the
+ * default) and schedule; 0 for invalid arguments. */
+int fteik3d_sweep_launches(int nz, int nx, int ny, int nsrc, int tile_a, int schedule);
+
 /* Bytes of the file-scope device scratch array of fteik3d_solve_noalloc (64 MiB). */
 size_t fteik3d_scratch_capacity(void);
 
@@ -91,7 +101,7 @@ size_t fteik3d_scratch_capacity(void);
  * hipMalloc).  The call allocates nothing: its workspace is a file-scope device array of
  * fteik3d_scratch_capacity() bytes that each device running the library holds until the
  * process ends (its address is looked up once per device on the first call there).
- * Arguments and results as fteik3d_solve with schedule 1 and use_graph 0; problems whose
+ * Arguments and results as fteik3d_solve with schedule 2 and use_graph 0; problems whose
  * fteik3d_workspace_size exceeds the capacity return FTEIK_ERR_WORKSPACE before anything
  * is enqueued.  All calls on one device share the array: calls that overlap on one device
  * (concurrent host threads, or different streams) race and must be serialized by the

@@ -127,6 +127,10 @@ def test_persistent_schedule_is_refused():
     st = L.fteik3d_solve(fake, 4, 4, 4, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, 0, 0, fake, None, fake,
                          fake, fake, ws, 0, None)
     assert st == _lib.ERR_UNSUPPORTED
+    for bad in (-1, 3):  # no such schedule
+        st = L.fteik3d_solve(fake, 4, 4, 4, 1.0, 1.0, 1.0, sp, 1, 2, 10, 0, 0, 0, bad, fake, None,
+                             fake, fake, fake, ws, 0, None)
+        assert st == _lib.ERR_ARG
 
 
 def test_adjoint_host_argument_checks():
@@ -290,3 +294,21 @@ def test_spacing_range():
         st = L.fteik2d_solve(fake, 4, 4, *d, sp2, 1, 2, 10, 0, fake, None, fake, fake, fake,
                              1 << 40, None)
         assert st == _lib.ERR_ARG, d
+
+
+def test_sweep_launches_per_set():
+    """Sweep-kernel launches per sweep set (host only): with the overlapped schedule 508
+    against 688 one pass at a time for 160 x 320 x 320 cells in the default 27-plane bricks
+    of 256 sources, 368 against 608 for 200^3 cells in one source's 8-plane bricks; bricks
+    under 8 planes run pass by pass; invalid arguments give 0."""
+    from fteikpy_amd import _lib
+
+    L = _lib.lib()
+    assert L.fteik3d_sweep_launches(160, 320, 320, 256, 0, 2) == 508
+    assert L.fteik3d_sweep_launches(160, 320, 320, 256, 0, 1) == 688
+    assert L.fteik3d_sweep_launches(200, 200, 200, 1, 0, 2) == 368
+    assert L.fteik3d_sweep_launches(200, 200, 200, 1, 0, 1) == 8 * (26 + 26 + 26 - 2)
+    assert L.fteik3d_sweep_launches(200, 200, 200, 1, 4, 2) == 8 * (51 + 26 + 26 - 2)
+    assert L.fteik3d_sweep_launches(160, 320, 320, 256, 0, 0) == 1
+    for bad in ((0, 4, 4, 1, 0, 2), (4, 4, 4, 0, 0, 2), (4, 4, 4, 1, -1, 2), (4, 4, 4, 1, 0, 3)):
+        assert L.fteik3d_sweep_launches(*bad) == 0

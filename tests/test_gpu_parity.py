@@ -104,6 +104,8 @@ PERSISTENT_OFF = pytest.mark.skip(
 @pytest.mark.parametrize("tile_a, group, schedule", [
     (1, 1, "diagonal"), (3, 2, "diagonal"), (8, 4, "diagonal"), (0, 0, "diagonal"),
     (64, 8, "diagonal"), (5, 4, "diagonal"),
+    (8, 2, "overlapped"), (9, 1, "overlapped"), (0, 0, "overlapped"), (64, 8, "overlapped"),
+    (10, 4, "overlapped"), (3, 2, "overlapped"),
     pytest.param(3, 2, "persistent", marks=PERSISTENT_OFF),
     pytest.param(0, 4, "persistent", marks=PERSISTENT_OFF)])
 def test_brick_shapes_bitwise(fa, tile_a, group, schedule):
@@ -133,6 +135,29 @@ def test_graph_mode_matches_direct(fa):
         b2 = eik.solve_tensors(srcs, nsweep=3, use_graph=True)
         a2 = eik.solve_tensors(srcs, nsweep=3, use_graph=False)
         assert np.array_equal(a2[0].cpu().numpy(), b2[0].cpu().numpy())
+
+
+def test_overlapped_equals_pass_by_pass(fa):
+    """The overlapped schedule (bricks of several passes in one launch) and one pass per
+    launch give identical traveltimes, upwind signs and set counts at every nsweep and
+    converged, directly and from a captured graph, and both equal the oracle."""
+    shape, d = (40, 45, 38), (9.0, 10.0, 11.0)
+    v = _rough_model(shape, 23)
+    rng = np.random.default_rng(23)
+    srcs = rng.uniform(0.0, 1.0, (6, 3)) * np.array(shape) * np.array(d)
+    eik = fa.Eikonal3D(v, d)
+    for ns in (1, 2, 3, None):
+        for tile, group, graph in ((8, 2, False), (0, 0, True), (12, 1, True)):
+            a = eik.solve_tensors(srcs, nsweep=ns, return_gradient=True, tile_a=tile,
+                                  group=group, schedule="diagonal")
+            b = eik.solve_tensors(srcs, nsweep=ns, return_gradient=True, tile_a=tile,
+                                  group=group, schedule="overlapped", use_graph=graph)
+            assert np.array_equal(a[0].cpu().numpy(), b[0].cpu().numpy()), (ns, tile)
+            assert np.array_equal(a[1].cpu().numpy(), b[1].cpu().numpy()), (ns, tile)
+            assert list(a[3]) == list(b[3]), (ns, tile)
+    ref, _, _, kref = oracle.solve3d(1.0 / v, *d, srcs, nsweep=0)
+    assert list(b[3]) == list(kref)
+    assert np.array_equal(b[0].cpu().numpy(), ref)
 
 
 def test_larger_model_bitwise(fa):
@@ -346,7 +371,9 @@ def test_many_sets_skip_bitwise(fa):
     ref, _, _, kref = oracle.solve3d(1.0 / v, *d, srcs, nsweep=0)
     assert kref.max() >= 10
     for tile, group, sched in ((0, 4, "diagonal"), (16, 1, "diagonal"), (7, 2, "diagonal"),
-                               (5, 8, "diagonal"), (1, 1, "diagonal")):
+                               (5, 8, "diagonal"), (1, 1, "diagonal"), (0, 4, "overlapped"),
+                               (8, 1, "overlapped"), (13, 2, "overlapped"),
+                               (16, 8, "overlapped")):
         tt, _, _, sets = eik.solve_tensors(srcs, nsweep=None, tile_a=tile, group=group,
                                            schedule=sched, use_graph=True)
         assert list(sets) == list(kref), (tile, group)
