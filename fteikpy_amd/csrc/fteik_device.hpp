@@ -91,6 +91,16 @@ __device__ __forceinline__ double sqrt_rn_lane(double x) {
     return g;
 }
 
+// Orders this wave's memory accesses across the call for the compiler; no hardware wait.
+// A wave's LDS instructions execute in order, so after it every lane of the wave sees what
+// the others wrote to LDS before it (the wave-synchronous exchange rocPRIM's wave_barrier
+// provides).  Called by every lane of the wave.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Converts a HIP error into the library's negative return code.
 inline int hip_status(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 

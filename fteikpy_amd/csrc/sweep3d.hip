@@ -144,6 +144,17 @@ __device__ __forceinline__ unsigned neighbour_byte(const SweepArgs &A, int s0, i
     return (unsigned)(A.faces[t] >> (8 * f)) & 0xffu;
 }
 
+// End of a sweep step (and of the ring's first fill): the next step reads ring entries
+// that other threads wrote.  In a one-wave workgroup they are all lanes of the same wave,
+// whose LDS instructions execute in order, so ordering the accesses for the compiler
+// suffices (no wait for the LDS writes to complete, no barrier); wider workgroups stage
+// entries across waves and wait at a workgroup barrier.
+template <int W>
+__device__ __forceinline__ void step_sync() {
+    if constexpr (W == 1) wave_sync();
+    else __syncthreads();
+}
+
 // pre < 0: the brick tests each wave's source for skipping itself; else bit w of pre says
 // whether wave w's source is active (the caller tested it, the same test).
 template <bool GRAD, bool LOG, int W>
@@ -328,7 +339,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
 #pragma unroll
     for (int p = -1; p < kPrefetch; ++p)
         if (p < na) store_plane(p, v0[p + 1]);
-    __syncthreads();
+    step_sync<W>();
     // From the second set on, a node none of whose 7 upwind neighbours changed during the
     // last 8 passes sees exactly the candidates of its previous update in this direction,
     // while its own time can only have decreased since: it cannot change and is skipped.
@@ -468,7 +479,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
             const int pl = s + kPrefetch;
             if (pl < na) store_plane(pl, rv[k]);
             if (pl + kDepth < na) load_plane(pl + kDepth, rv[k]);
-            __syncthreads();
+            step_sync<W>();
         }
     }
 
