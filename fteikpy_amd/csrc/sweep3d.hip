@@ -532,13 +532,16 @@ void sweep3d_diag_kernel(const SweepArgs A) {
     brick<GRAD, LOG, W>(A, pass, BA, BB, BC, blockIdx.y * W, -1);
 }
 
-// Batch of models: one launch per (pass, anti-diagonal d) covers the bricks of d of every
+// Batch of models: launch l of a sweep set covers the bricks of launch l of every
 // model.  cum[m] = first block of model m (non-decreasing, cum[nmodel] = grid size); a
-// model's blocks enumerate (brick on d, source group), source group fastest.
+// model's blocks enumerate (brick of this launch, source group), source group fastest; the
+// bricks of model m in this launch are tasks[toff[m] ...] (packed pass and oriented brick,
+// each model's overlapped schedule).
 template <bool GRAD, int W>
 __global__ __launch_bounds__(64 * W) void sweep3d_multi_kernel(const SweepArgs *models,
-                                                               const int *cum, int nmodel,
-                                                               int pass, int diag) {
+                                                               const int *cum, const int *toff,
+                                                               const unsigned long long *tasks,
+                                                               int nmodel) {
     const int b = blockIdx.x;
     int lo;  // the last model whose first block is <= b (cum is non-decreasing, cum[0] = 0)
     if (nmodel <= 64) {
@@ -558,8 +561,8 @@ __global__ __launch_bounds__(64 * W) void sweep3d_multi_kernel(const SweepArgs *
     const SweepArgs A = models[lo];
     const int local = b - cum[lo];
     const int t = local / A.gmax, g = local - t * A.gmax;
-    int BA, BB, BC;
-    decode_brick(diag, t, A.nba, A.nbb, A.nbc, &BA, &BB, &BC);
+    int pass, BA, BB, BC;
+    sched::unpack_task(tasks[toff[lo] + t], &pass, &BA, &BB, &BC);
     brick<GRAD, false, W>(A, pass, BA, BB, BC, g * W, -1);
 }
 
@@ -1422,18 +1425,21 @@ namespace {
 
 template <bool G, int W>
 void launch_multi_t(unsigned blocks, hipStream_t st, const SweepArgs *models, const int *cum,
-                    int nmodel, int pass, int diag) {
+                    const int *toff, const unsigned long long *tasks, int nmodel) {
     hipLaunchKernelGGL((sweep3d_multi_kernel<G, W>), dim3(blocks), dim3(64 * W), 0, st, models,
-                       cum, nmodel, pass, diag);
+                       cum, toff, tasks, nmodel);
 }
 
 // Device layout of a batch's 3D part: model arguments, bookkeeping arguments, the active
-// source count of every model (one int each, read back together), the block offsets of
-// every (anti-diagonal, model), then each model's own solve workspace.
+// source count of every model (one int each, read back together), the block offsets and
+// task offsets of every (launch, model), every model's overlapped brick list, then each
+// model's own solve workspace.  Launches of a set: at most 8 x the longest anti-diagonal
+// count (the overlapped schedule never needs more than one pass at a time).
 struct Batch3Layout {
-    size_t args, book, nlist, cum, models, total;
+    size_t args, book, nlist, cum, toff, tasks, models, total;
     std::vector<size_t> model_off;
     int dmax;
+    long long ntasks;
 };
 
 // Brick height of a batch's 3D model: its own tile_a, else the default for the batch's
@@ -1450,6 +1456,7 @@ int batch_tile(const fteik_problem *p, int n, const fteik_problem &q) {
 
 void batch3_layout(const fteik_problem *p, int n, Batch3Layout *L) {
     int n3 = 0, dmax = 0;
+    long long ntasks = 0;
     L->model_off.clear();
     size_t off = 0;
     std::vector<int> idx;
@@ -1458,11 +1465,15 @@ void batch3_layout(const fteik_problem *p, int n, Batch3Layout *L) {
             idx.push_back(i);
             ++n3;
             const int NZ = p[i].nz + 1;
-            const int tile = std::min(batch_tile(p, n, p[i]), NZ);
-            dmax = std::max(dmax, ceil_div(NZ, tile) + ceil_div(p[i].nx + 1, kLanesB) +
-                                      ceil_div(p[i].ny + 1, kLanesC) - 2);
+            const int tile = std::max(1, std::min(batch_tile(p, n, p[i]), NZ));
+            const long long nba = ceil_div(NZ, tile), nbb = ceil_div(p[i].nx + 1, kLanesB),
+                            nbc = ceil_div(p[i].ny + 1, kLanesC);
+            dmax = std::max(dmax, (int)(nba + nbb + nbc - 2));
+            ntasks += 8 * nba * nbb * nbc;
         }
     L->dmax = dmax;
+    L->ntasks = ntasks;
+    const size_t rows = (size_t)8 * dmax * (n3 + 1);
     L->args = off;
     off += align256(sizeof(SweepArgs) * n3);
     L->book = off;
@@ -1470,7 +1481,11 @@ void batch3_layout(const fteik_problem *p, int n, Batch3Layout *L) {
     L->nlist = off;
     off += align256(sizeof(int) * n3);
     L->cum = off;
-    off += align256(sizeof(int) * (size_t)dmax * (n3 + 1));
+    off += align256(sizeof(int) * rows);
+    L->toff = off;
+    off += align256(sizeof(int) * rows);
+    L->tasks = off;
+    off += align256(sizeof(unsigned long long) * (size_t)ntasks);
     L->models = off;
     for (int i : idx) {
         L->model_off.push_back(off);
@@ -1486,7 +1501,9 @@ struct Batch3 {
     std::vector<BookArgs> book;
     std::vector<std::vector<SrcParam>> prm;
     std::vector<Ws3> ws;
-    std::vector<int> cum;          // [dmax][n3 + 1]
+    std::vector<int> cum, toff;    // [launch][n3 + 1]: first block, first task of each model
+    std::vector<unsigned long long> tasks;  // every model's brick list, concatenated
+    int launches = 0;
     std::vector<const fteik_problem *> probs;
     Batch3Layout L;
     char *base = nullptr;
@@ -1556,19 +1573,44 @@ int batch3_prepare(const fteik_problem *P, int n, int nsweep, int max_sets, int 
         ++k;
     }
     const int n3 = k;
-    b->cum.assign((size_t)b->L.dmax * (n3 + 1), 0);
-    for (int d = 0; d < b->L.dmax; ++d) {
+    // every model sweeps on its own overlapped schedule; launch l of the batch runs launch l
+    // of every model that has one
+    std::vector<std::shared_ptr<const sched::OverlapTable>> tab(n3);
+    std::vector<long long> base(n3);
+    int nl = 0;
+    for (int m = 0; m < n3; ++m) {
+        const SweepArgs &a = b->args[m];
+        tab[m] = overlap_table_for(a.nba, a.nbb, a.nbc);
+        if (!tab[m]) {
+            delete b;
+            return FTEIK_ERR_ARG;
+        }
+        base[m] = (long long)b->tasks.size();
+        b->tasks.insert(b->tasks.end(), tab[m]->task.begin(), tab[m]->task.end());
+        nl = std::max(nl, tab[m]->launches());
+    }
+    if ((long long)b->tasks.size() != b->L.ntasks || (long long)b->tasks.size() >= (1LL << 31) ||
+        nl > 8 * b->L.dmax) {
+        delete b;
+        return FTEIK_ERR_ARG;
+    }
+    b->launches = nl;
+    b->cum.assign((size_t)nl * (n3 + 1), 0);
+    b->toff.assign((size_t)nl * (n3 + 1), 0);
+    for (int l = 0; l < nl; ++l) {
         long long c = 0;
         for (int m = 0; m < n3; ++m) {
-            b->cum[(size_t)d * (n3 + 1) + m] = (int)c;
-            const SweepArgs &a = b->args[m];
-            if (d < a.ndiag) c += (long long)bricks_on_diag(d, a.nba, a.nbb, a.nbc) * a.gmax;
+            const size_t i = (size_t)l * (n3 + 1) + m;
+            b->cum[i] = (int)c;
+            const sched::OverlapTable &t = *tab[m];
+            b->toff[i] = (int)(base[m] + (l < t.launches() ? t.first[l] : 0));
+            if (l < t.launches()) c += (long long)(t.first[l + 1] - t.first[l]) * b->args[m].gmax;
         }
         if (c >= (1LL << 31)) {
             delete b;
             return FTEIK_ERR_ARG;
         }
-        b->cum[(size_t)d * (n3 + 1) + n3] = (int)c;
+        b->cum[(size_t)l * (n3 + 1) + n3] = (int)c;
     }
     *out = b;
     return 0;
@@ -1584,11 +1626,18 @@ int batch3_run(Batch3 *b, int use_graph, hipStream_t stream) {
     BookArgs *book_dev = (BookArgs *)(ws + b->L.book);
     int *nlist_dev = (int *)(ws + b->L.nlist);
     int *cum_dev = (int *)(ws + b->L.cum);
+    int *toff_dev = (int *)(ws + b->L.toff);
+    unsigned long long *tasks_dev = (unsigned long long *)(ws + b->L.tasks);
     FTEIK_CHECK(hipMemcpyAsync(args_dev, b->args.data(), sizeof(SweepArgs) * n3,
                                hipMemcpyHostToDevice, stream));
     FTEIK_CHECK(hipMemcpyAsync(book_dev, b->book.data(), sizeof(BookArgs) * n3,
                                hipMemcpyHostToDevice, stream));
     FTEIK_CHECK(hipMemcpyAsync(cum_dev, b->cum.data(), sizeof(int) * b->cum.size(),
+                               hipMemcpyHostToDevice, stream));
+    FTEIK_CHECK(hipMemcpyAsync(toff_dev, b->toff.data(), sizeof(int) * b->toff.size(),
+                               hipMemcpyHostToDevice, stream));
+    FTEIK_CHECK(hipMemcpyAsync(tasks_dev, b->tasks.data(),
+                               sizeof(unsigned long long) * b->tasks.size(),
                                hipMemcpyHostToDevice, stream));
     for (int m = 0; m < n3; ++m)
         FTEIK_CHECK(hipMemcpyAsync(b->ws[m].prm, b->prm[m].data(),
@@ -1621,15 +1670,14 @@ int batch3_run(Batch3 *b, int use_graph, hipStream_t stream) {
             return st;
     }
 
-    const int dmax = b->L.dmax;
     auto enqueue_set = [&]() -> int {
-        for (int p = 0; p < 8; ++p)
-            for (int d = 0; d < dmax; ++d) {
-                const int blocks = b->cum[(size_t)d * (n3 + 1) + n3];
-                if (blocks == 0) continue;
-                FTEIK_DISPATCH_W(launch_multi_t, b->grad, b->group, (unsigned)blocks, stream,
-                                 args_dev, cum_dev + (size_t)d * (n3 + 1), n3, p, d);
-            }
+        for (int l = 0; l < b->launches; ++l) {
+            const int blocks = b->cum[(size_t)l * (n3 + 1) + n3];
+            if (blocks == 0) continue;
+            FTEIK_DISPATCH_W(launch_multi_t, b->grad, b->group, (unsigned)blocks, stream,
+                             args_dev, cum_dev + (size_t)l * (n3 + 1),
+                             toff_dev + (size_t)l * (n3 + 1), tasks_dev, n3);
+        }
         hipLaunchKernelGGL(set_bookkeeping_multi_kernel, dim3(n3), dim3(256), 0, stream,
                            book_dev);
         return hip_status(hipGetLastError());
