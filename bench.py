@@ -146,12 +146,13 @@ def main():
     tune = None
     with torch.cuda.stream(stream):
         if not args.no_tune and not args.tile and not args.group:
-            # brick height and group for this workload, timed on it before the warm-up
+            # brick height, group and schedule for this workload, timed on it before the
+            # warm-up; the solves below leave all three at their defaults and so use them
             tune = fteikpy_amd.autotune.tune3d(eik, src, tiles=(16, 32, 0),
                                                use_graph=use_graph)
         if tune:
-            log(f"autotune: tile_a {tune['tile_a']} group {tune['group']} "
-                f"({tune['seconds']:.1f} s)")
+            log(f"autotune: tile_a {tune['tile_a']} group {tune['group']} schedule "
+                f"{tune['schedule']} ({tune['seconds']:.1f} s)")
         out = torch.empty((nsrc, nz + 1, nx + 1, ny + 1), dtype=torch.float64, device=dev)
         for _ in range(args.warmup):
             eik.solve_tensors(src, nsweep=None, tile_a=args.tile, group=args.group,
@@ -206,7 +207,8 @@ def main():
         ana = None if args.no_analytic else analytic_errors(torch, fteikpy_amd, dev)
         c2 = None if args.no_analytic else single_source_c2(torch, fteikpy_amd, synthetic, dev,
                                                             with_cpu=not args.no_cpu)
-        ub = None if args.no_batch else uneven_batch(torch, fteikpy_amd, synthetic, dev)
+        ub = None if args.no_batch else uneven_batch(torch, fteikpy_amd, synthetic, dev,
+                                                     tune=not args.no_tune)
         p2d = None if args.no_analytic else plain_2d(torch, fteikpy_amd, synthetic, dev,
                                                      with_cpu=not args.no_cpu)
 
@@ -219,7 +221,9 @@ def main():
     # overlapped schedule), until its last source converged
     tile = args.tile or (tune["tile_a"] if tune else 0)
     group = args.group or (tune["group"] if tune else (2 if nsrc >= 8 else 1))
-    per_set = fteikpy_amd._lib.lib().fteik3d_sweep_launches(nz, nx, ny, nsrc, tile, 2)
+    sched = tune["schedule"] if tune else "overlapped"
+    per_set = fteikpy_amd._lib.lib().fteik3d_sweep_launches(
+        nz, nx, ny, nsrc, tile, {"overlapped": 2, "diagonal": 1}[sched])
     launches = args.steps * int(k_sets.max()) * per_set
     traffic = None
     if (tuple(shape) == PMC_TRAFFIC["grid"] and nsrc == PMC_TRAFFIC["sources"]
@@ -271,7 +275,8 @@ def main():
                    "sets_per_source": [int(k_sets.min()), int(k_sets.max())],
                    "hipgraph": use_graph,
                    "tile_a": args.tile or (tune["tile_a"] if tune else 0),
-                   "group": args.group or (tune["group"] if tune else 0)},
+                   "group": args.group or (tune["group"] if tune else 0),
+                   "schedule": sched},
         "autotune": tune,
         "plain_value": nplain / plain_s if plain_s else None,
         "plain_sources_timed": nplain,
@@ -568,15 +573,23 @@ def plain_2d(torch, fa, synthetic, dev, with_cpu=True, reps=3):
     return out
 
 
-def uneven_batch(torch, fa, synthetic, dev, reps=3):
+def uneven_batch(torch, fa, synthetic, dev, reps=3, tune=False):
     """The uneven batch of independent models (16 2D models 64 to 1024 per side and 32 3D
     models 48^3 to 192 x 256 x 256, 1 to 32 sources each, smooth random velocities),
     converged in one fteik_solve_batch call; beside it the 3D and the 2D models alone and
-    the 3D models solved one call each (best of `reps` after a warm-up, wall time)."""
+    the 3D models solved one call each (best of `reps` after a warm-up, wall time).  With
+    `tune`, the batch autotuner picks every 3D model's brick height first (recorded for the
+    whole batch and for its 3D part) and the batch calls use them."""
     models = synthetic.uneven_batch()
     probs = [((fa.Eikonal3D if v.ndim == 3 else fa.Eikonal2D)(v, d), s) for v, d, s in models]
     p3 = [p for p in probs if isinstance(p[0], fa.Eikonal3D)]
     p2 = [p for p in probs if not isinstance(p[0], fa.Eikonal3D)]
+    tuned = None
+    if tune:
+        tuned = fa.autotune.tune_batch(probs)
+        fa.autotune.record_batch(p3, [t for (e, _), t in zip(probs, tuned["tiles"])
+                                      if isinstance(e, fa.Eikonal3D)])
+        log(f"batch autotune: heights {tuned['tiles']} ({tuned['seconds']:.1f} s)")
 
     slowest = {}
 
@@ -611,7 +624,7 @@ def uneven_batch(torch, fa, synthetic, dev, reps=3):
             "seconds_3d_one_call_each": tloop, "batched_3d_over_one_call_each": tloop / t3,
             "batched_equals_one_call_each_bitwise": equal,
             "max_sets_2d": max(sets[:off]), "max_sets_3d": max(sets[off:]),
-            "slowest_repetition_s": slowest,
+            "slowest_repetition_s": slowest, "autotune": tuned,
             "note": f"converged (nsweep=None); wall time, best of {reps} repetitions after a "
                     "warm-up (the 3D one-call-each loop: 1); the 2D models run on their own "
                     "stream beside the 3D sweeps"}

@@ -3,10 +3,10 @@
 No single counterpart in the reference, which solves one model per
 ``Eikonal2D/3D.solve`` call (fteikpy/_solver.py:29-77, :101-149).  ``solve_batch`` takes
 (solver, sources) pairs and returns, for each, exactly what ``solver.solve(sources, ...)``
-returns -- bit for bit -- but all 3D models share every sweep launch (one launch per
-(pass, brick anti-diagonal) covers the bricks and active sources of every model) and the
-2D models run meanwhile on a stream of their own, one wave per source.  C entry point:
-``fteik_solve_batch``.
+returns -- bit for bit -- but all 3D models share every sweep launch (each model sweeps on
+its own overlapped schedule, and launch l of a set covers launch l's bricks and active
+sources of every model) and the 2D models run meanwhile on a stream of their own, one wave
+per source.  C entry point: ``fteik_solve_batch``.
 """
 import ctypes
 
@@ -23,13 +23,16 @@ def solve_batch_tensors(problems, nsweep=2, return_gradient=False, *, max_sets=1
 
     Returns one (tt, signs or None, vzero (numpy), sets executed (numpy)) tuple per problem,
     tt of shape [nsrc, nz+1, nx+1(, ny+1)] on the device.  `tiles` optionally gives the 3D
-    brick height per problem (0 = the autotuner's recorded choice or the library default)."""
+    brick height per problem (0 = the library's rule for a batch); left unset, the heights
+    ``autotune.tune_batch`` recorded for this batch apply, else the library's rule."""
     torch = _torch()
     dev = device()
     L = _lib.lib()
     n = len(problems)
     if n == 0:
         return []
+    if tiles is None:
+        tiles = autotune.lookup_batch(problems)
     arr = (_lib.Problem * n)()
     keep, outs = [], []
     for i, (eik, sources) in enumerate(problems):
@@ -46,9 +49,7 @@ def solve_batch_tensors(problems, nsweep=2, return_gradient=False, *, max_sets=1
         vz = torch.empty(ns, dtype=torch.float64, device=dev)
         sets = torch.empty(ns, dtype=torch.int32, device=dev)
         slow = eik._slowness()
-        tile = int(tiles[i]) if tiles is not None else 0
-        if nd == 3 and not tile:
-            tile = autotune.lookup3d(shape, ns)[0]
+        tile = int(tiles[i]) if tiles is not None and nd == 3 else 0
         p = arr[i]
         p.ndim = nd
         p.nz, p.nx = shape[0], shape[1]

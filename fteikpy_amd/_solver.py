@@ -6,13 +6,14 @@ for a batch of sources).  Keyword-only extensions:
 - ``nsweep=None`` (or <= 0): sweep until a set changes no node, at most ``max_sets`` sets;
 - ``tile_a``: z-extent of the 3D sweep bricks (0 = library default);
 - ``group``: sources per workgroup (0 = default; 1, 2, 4, 8);
-  with both left at 0, a choice recorded by ``fteikpy_amd.autotune.tune3d`` for this grid
-  shape and number of sources is used;
-- ``schedule``: "overlapped" (the default: each launch runs the bricks, of any pass, whose
-  inputs are final, so a pass starts while the previous one finishes its far side; bricks
-  under 8 planes run pass by pass), "diagonal" (one launch per brick anti-diagonal and
-  pass) or "persistent" (one launch per sweep set; currently refused); the results are
-  identical;
+  with both left at 0 (and ``schedule`` at None), a choice recorded by
+  ``fteikpy_amd.autotune.tune3d`` for this grid shape and number of sources is used;
+- ``schedule``: "overlapped" (each launch runs the bricks, of any pass, whose inputs are
+  final, so a pass starts while the previous one finishes its far side; bricks under 8
+  planes run pass by pass), "diagonal" (one launch per brick anti-diagonal and pass) or
+  "persistent" (one launch per sweep set; currently refused); the results are identical;
+  None (the default): the autotuner's recorded choice when ``tile_a`` and ``group`` are
+  left at 0 too, else "overlapped";
 - ``use_graph``: capture one 3D sweep set as a hipGraph and replay it.
 """
 import ctypes
@@ -93,7 +94,7 @@ class Eikonal3D(BaseGrid3D, _SlownessCache):
                          origin=origin if origin is not None else np.zeros(3, dtype=np.float64))
 
     def solve(self, sources, nsweep=2, return_gradient=False, *, max_sets=100, tile_a=0,
-              group=0, schedule="overlapped", use_graph=False):
+              group=0, schedule=None, use_graph=False):
         """Solve the eikonal equation for one source [z, x, y] or a list of sources."""
         tt, sgn, vz, nsets = self.solve_tensors(sources, nsweep, return_gradient,
                                                 max_sets=max_sets, tile_a=tile_a, group=group,
@@ -109,7 +110,7 @@ class Eikonal3D(BaseGrid3D, _SlownessCache):
         return grids[0] if single else grids
 
     def solve_tensors(self, sources, nsweep=2, return_gradient=False, *, max_sets=100, tile_a=0,
-                      group=0, schedule="overlapped", use_graph=False, out=None):
+                      group=0, schedule=None, use_graph=False, out=None):
         """Batched solve returning device tensors: (tt [n, nz+1, nx+1, ny+1], signs or None,
         vzero (numpy [n]), sets executed (numpy [n])).  `out` may pass a preallocated tt."""
         torch = _torch()
@@ -135,8 +136,10 @@ class Eikonal3D(BaseGrid3D, _SlownessCache):
         wsb = L.fteik3d_workspace_size(nz, nx, ny, n)
         ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
         slow = self._slowness()
-        if not tile_a and not group:
-            tile_a, group = autotune.lookup3d(self.shape, n)
+        if not tile_a and not group and schedule is None:
+            tile_a, group, schedule = autotune.choice3d(self.shape, n)
+        if schedule is None:
+            schedule = "overlapped"
         args = (_ptr(slow), nz, nx, ny, *self._gridsize, rp, n, _nsweep(nsweep), int(max_sets),
                 int(bool(return_gradient)), int(tile_a), int(group),
                 _SCHEDULES[schedule], _ptr(tt),

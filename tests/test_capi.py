@@ -252,11 +252,38 @@ def test_autotune_records_and_persists(tmp_path):
     autotune.record3d((10, 20, 30), 4, 16, 2)
     assert autotune.lookup3d((10, 20, 30), 4) == (16, 2)
     assert autotune.lookup3d((10, 20, 30), 5) == (0, 0)
+    assert autotune.choice3d((10, 20, 30), 4) == (16, 2, None)
+    autotune.record3d((10, 20, 30), 8, 24, 1, "diagonal")
+    assert autotune.choice3d((10, 20, 30), 8) == (24, 1, "diagonal")
+    with pytest.raises(ValueError):
+        autotune.record3d((10, 20, 30), 8, 24, 1, "persistent")
+
+    class _Eik:  # stand-ins: the batch key needs each problem's shape and source count
+        def __init__(self, shape):
+            self.shape = shape
+
+    probs = [(_Eik((10, 20, 30)), np.zeros((3, 3))), (_Eik((40, 50)), np.zeros(2)),
+             (_Eik((6, 7, 8)), np.zeros(3))]
+    assert autotune.batch_signature(probs) == "3d10x20x30:3;2d40x50:1;3d6x7x8:1"
+    assert autotune.lookup_batch(probs) is None
+    autotune.record_batch(probs, [16, 0, 8])
+    assert autotune.lookup_batch(probs) == [16, 0, 8]
+    assert autotune.lookup_batch(probs[:2]) is None
+    with pytest.raises(ValueError):
+        autotune.record_batch(probs, [16, 0])
     path = str(tmp_path / "tune.json")
     autotune.save(path)
     autotune.clear()
     autotune.load(path)
     assert autotune.lookup3d((10, 20, 30), 4) == (16, 2)
+    assert autotune.choice3d((10, 20, 30), 8) == (24, 1, "diagonal")
+    assert autotune.lookup_batch(probs) == [16, 0, 8]
+    autotune.clear()
+    # a file of (tile_a, group) pairs from before schedules were tuned still loads
+    old = tmp_path / "old.json"
+    old.write_text('{"10x20x30:4": [16, 2]}')
+    autotune.load(str(old))
+    assert autotune.choice3d((10, 20, 30), 4) == (16, 2, None)
     autotune.clear()
 
 
