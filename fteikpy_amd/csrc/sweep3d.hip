@@ -839,6 +839,27 @@ using sched::bricks_on_diag;
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
+// Source groups (grid.y) of every sweep launch of an instantiated set graph, from old_gy to
+// gy: the kernel nodes whose grid.y is old_gy, other than the set's bookkeeping kernel
+// (one block).
+int set_graph_groups(hipGraph_t graph, hipGraphExec_t exec, unsigned old_gy, unsigned gy) {
+    size_t n = 0;
+    FTEIK_CHECK(hipGraphGetNodes(graph, nullptr, &n));
+    std::vector<hipGraphNode_t> nodes(n);
+    FTEIK_CHECK(hipGraphGetNodes(graph, nodes.data(), &n));
+    for (size_t i = 0; i < n; ++i) {
+        hipGraphNodeType t;
+        FTEIK_CHECK(hipGraphNodeGetType(nodes[i], &t));
+        if (t != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeParams p;
+        FTEIK_CHECK(hipGraphKernelNodeGetParams(nodes[i], &p));
+        if (p.gridDim.y != old_gy || (p.gridDim.x == 1 && p.gridDim.y == 1)) continue;
+        p.gridDim.y = gy;
+        FTEIK_CHECK(hipGraphExecKernelNodeSetParams(exec, nodes[i], &p));
+    }
+    return 0;
+}
+
 // A captured sweep set, destroyed on every return path.
 struct Graphs {
     hipGraph_t graph = nullptr;
@@ -1207,6 +1228,10 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
         }
     }
 
+    // source groups launched per brick: all of them, until a converging solve finds that at
+    // most half are still active (their sources compacted to the front of the list), then
+    // those (the others' workgroups would only read the count and leave)
+    unsigned gy = (unsigned)gmax;
     auto enqueue_set = [&]() -> int {
         if (schedule == 0) {
             const long long np = (long long)a.ntiles * gmax;
@@ -1218,7 +1243,7 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
             for (int l = 0; l < ov->launches(); ++l) {
                 a.diag = ov->first[l];
                 FTEIK_DISPATCH_W(launch_diag_t, want_grad != 0, group,
-                                 dim3(ov->first[l + 1] - ov->first[l], (unsigned)gmax), stream, a);
+                                 dim3(ov->first[l + 1] - ov->first[l], gy), stream, a);
             }
         } else {
             for (int p = 0; p < 8; ++p) {
@@ -1226,11 +1251,11 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
                 for (int d = 0; d < a.ndiag; ++d) {
                     a.diag = d;
                     if (hv) {
-                        FTEIK_DISPATCH_W(launch_diag_log_t, false, group,
-                                         dim3(nb[d], (unsigned)gmax), stream, a);
+                        FTEIK_DISPATCH_W(launch_diag_log_t, false, group, dim3(nb[d], gy),
+                                         stream, a);
                     } else {
-                        FTEIK_DISPATCH_W(launch_diag_t, want_grad != 0, group,
-                                         dim3(nb[d], (unsigned)gmax), stream, a);
+                        FTEIK_DISPATCH_W(launch_diag_t, want_grad != 0, group, dim3(nb[d], gy),
+                                         stream, a);
                     }
                 }
                 if (hv)
@@ -1271,6 +1296,11 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
             hipError_t e = hipMemcpyAsync(&nact, w.nlist, sizeof(int), hipMemcpyDeviceToHost, stream);
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
             status = hip_status(e);
+            const unsigned need = (unsigned)std::max(1, ceil_div(nact, group));
+            if (!status && schedule != 0 && nact > 0 && 2 * need <= gy) {
+                if (gr.exec) status = set_graph_groups(gr.graph, gr.exec, gy, need);
+                gy = need;
+            }
         }
     }
     if (!status && schedule == 0) {
