@@ -378,7 +378,7 @@ def row_calibration(reps=8, shape=(161, 321, 321), first=7):
                       "checksum": float(out.sum())}), flush=True)
 
 
-def headline_post(torch, fa, eik, src, rec, d, nrays=64):
+def headline_post(torch, fa, eik, src, rec, d, nrays=64, chunk=32):
     """The rest of the batched configuration: the converged solve again with gradient
     signs, then per source its gradient grids, traveltimes at the 4096 receivers and
     `nrays` ray paths (to the first receivers), one source after another (wall time)."""
@@ -387,23 +387,44 @@ def headline_post(torch, fa, eik, src, rec, d, nrays=64):
     tt, sgn, vz, sets = eik.solve_tensors(src, nsweep=None, return_gradient=True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    npts = 0
+    npts, first = 0, None
     for i in range(len(src)):
         g = fa.TraveltimeGrid3D(tt[i], d, np.zeros(3), src[i], None, float(vz[i]), sign=sgn[i])
         g(rec)
         rays = g.raytrace(rec[:nrays])
         npts += sum(len(r) for r in rays)
+        if i == 0:
+            first = rays
         del g
     torch.cuda.synchronize()
     t2 = time.perf_counter()
+    # the same with the rays of `chunk` sources traced in one launch (raytrace_batch); the
+    # receivers stay one call per source
+    npts_b, equal = 0, None
+    for c0 in range(0, len(src), chunk):
+        grids = [fa.TraveltimeGrid3D(tt[i], d, np.zeros(3), src[i], None, float(vz[i]),
+                                     sign=sgn[i]) for i in range(c0, min(len(src), c0 + chunk))]
+        for g in grids:
+            g(rec)
+        rays = fa.raytrace_batch(grids, [rec[:nrays]] * len(grids))
+        npts_b += sum(len(r) for rr in rays for r in rr)
+        if c0 == 0:
+            equal = all(np.array_equal(a, b) for a, b in zip(rays[0], first))
+        del grids, rays
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
     del tt, sgn
-    log(f"headline post: solve with signs {t1 - t0:.2f} s, receivers and rays {t2 - t1:.2f} s")
+    log(f"headline post: solve with signs {t1 - t0:.2f} s, receivers and rays {t2 - t1:.2f} s "
+        f"(rays batched by {chunk}: {t3 - t2:.2f} s)")
     return {"solve_with_signs_s": t1 - t0, "receivers_and_rays_s": t2 - t1,
+            "receivers_and_rays_batched_s": t3 - t2, "rays_batch_sources": chunk,
+            "batched_rays_equal_bitwise": bool(equal) and npts_b == npts,
             "receivers_per_source": len(rec), "rays_per_source": nrays,
             "ray_points": npts,
             "note": "per source: gradient grids on the device (from the signs), "
                     "traveltimes at its receivers, rays to the first receivers (returned "
-                    "to the host as the reference's lists of arrays)"}
+                    "to the host as the reference's lists of arrays); batched: the rays of "
+                    f"{chunk} sources per raytrace_batch call (one launch), the same rays"}
 
 
 def single_source_c2(torch, fa, synthetic, dev, reps=3, with_cpu=True):

@@ -7,7 +7,7 @@ include/fteik_amd.h; arrays live in device memory as torch tensors.
 from . import autotune
 from ._adjoint import receiver_traveltimes
 from ._batch import solve_batch, solve_batch_tensors
-from ._grid import Grid2D, Grid3D, TraveltimeGrid2D, TraveltimeGrid3D
+from ._grid import Grid2D, Grid3D, TraveltimeGrid2D, TraveltimeGrid3D, raytrace_batch
 from ._helpers import get_num_threads, set_num_threads
 from ._io import grid_to_meshio, ray_to_meshio
 from ._lib import build as _build_lib
@@ -35,6 +35,7 @@ __all__ = [
     "receiver_traveltimes",
     "solve_batch",
     "solve_batch_tensors",
+    "raytrace_batch",
     "autotune",
     "build",
     "__version__",

@@ -348,11 +348,7 @@ class BaseTraveltime:
         torch = _torch()
         nd = self._ndim
         grad = self._gradient_tensor()
-        if honor_grid or not stepsize:
-            stepsize = np.min(self._gridsize)
-        if not max_step:
-            max_dist = 2.0 * sum((n * d) ** 2 for n, d in zip(self.shape, self._gridsize)) ** 0.5
-            max_step = int(max_dist / stepsize)
+        stepsize, max_step = _ray_steps(self, stepsize, max_step, honor_grid)
         if tuple(grad.shape[:-1]) != self.shape:
             raise ValueError("the gradient grid belongs to the solved traveltime grid, which "
                              "was resampled since: rays need the solved shape")
@@ -377,12 +373,96 @@ class BaseTraveltime:
         for code in (_lib.ERR_ENDPOINT_OOB, _lib.ERR_MAX_STEP):
             if (status == code).any():
                 _lib.check(code)
-        counts = counts.cpu().numpy()
         # only the used prefix of each ray comes back; returned source first
-        lmax = int(counts.max()) + 1 if n else 0
-        host = rays[:, :lmax].cpu().numpy()
-        out = [host[r, counts[r]::-1].copy() for r in range(n)]
+        out = _ray_lists(rays, counts.cpu().numpy(), n)
         return out[0] if single else out
+
+
+def _ray_steps(grid, stepsize, max_step, honor_grid):
+    """The step length and step budget TraveltimeGrid.raytrace uses (fteikpy/_grid.py:
+    103-145, :221-265)."""
+    if honor_grid or not stepsize:
+        stepsize = np.min(grid._gridsize)
+    if not max_step:
+        max_dist = 2.0 * sum((n * d) ** 2 for n, d in zip(grid.shape, grid._gridsize)) ** 0.5
+        max_step = int(max_dist / stepsize)
+    return float(stepsize), int(max_step)
+
+
+def _ray_lists(rays, counts, n):
+    """The used prefix of each ray in the work buffer, returned source first."""
+    lmax = int(counts.max()) + 1 if n else 0
+    host = rays[:, :lmax].cpu().numpy()
+    return [host[r, counts[r]::-1].copy() for r in range(n)]
+
+
+def raytrace_batch(grids, points, stepsize=None, max_step=None, honor_grid=False):
+    """Rays of several 3D traveltime grids of one model in one launch.
+
+    No counterpart in the reference, which traces one grid's rays per
+    ``TraveltimeGrid3D.raytrace`` call (fteikpy/_grid.py:221-265).  `grids`: traveltime
+    grids of the same shape, spacing and origin (one per source, as a batched solve returns
+    them); `points`: one entry per grid, its end points ([z, x, y] or an array of them).
+    Returns, per grid, what ``grid.raytrace(points_i, stepsize, max_step, honor_grid)``
+    returns, bit for bit, with every grid's rays traced together (C entry point
+    ``fteik3d_raytrace_batch``).  Each grid's gradient grids are materialised on the device
+    (as ``raytrace`` does); for many sources of a large model, pass the grids in chunks."""
+    torch = _torch()
+    grids = list(grids)
+    if not grids:
+        return []
+    if not all(isinstance(g, TraveltimeGrid3D) for g in grids):
+        raise TypeError("raytrace_batch traces TraveltimeGrid3D grids")
+    g0 = grids[0]
+    for g in grids[1:]:
+        if (g.shape != g0.shape or not np.array_equal(g._gridsize, g0._gridsize)
+                or not np.array_equal(g._origin, g0._origin)):
+            raise ValueError("raytrace_batch needs grids of one model (same shape, gridsize "
+                             "and origin)")
+    if len(points) != len(grids):
+        raise ValueError("raytrace_batch needs one entry of end points per grid")
+    pts = [np.asarray(p, dtype=np.float64) for p in points]
+    stepsize, max_step = _ray_steps(g0, stepsize, max_step, honor_grid)
+    grads = [g._gradient_tensor().contiguous() for g in grids]
+    for g, gr in zip(grids, grads):
+        if tuple(gr.shape[:-1]) != g.shape:
+            raise ValueError("the gradient grid belongs to the solved traveltime grid, which "
+                             "was resampled since: rays need the solved shape")
+    flat = [np.ascontiguousarray(p.reshape(-1, 3)) for p in pts]
+    sizes = [len(f) for f in flat]
+    n = int(sum(sizes))
+    dev = device()
+    if n == 0:
+        return [[] for _ in pts]
+    pd = torch.as_tensor(np.concatenate(flat), device=dev)
+    which = torch.as_tensor(np.repeat(np.arange(len(grids), dtype=np.int32), sizes), device=dev)
+    gptr = torch.as_tensor(np.array([gr.data_ptr() for gr in grads], dtype=np.uint64)
+                           .view(np.int64), device=dev)
+    srcs = torch.as_tensor(np.stack([np.asarray(g._source, dtype=np.float64) for g in grids]),
+                           device=dev)
+    rays = torch.empty((n, max_step, 3), dtype=torch.float64, device=dev)
+    counts = torch.empty(n, dtype=torch.int32, device=dev)
+    status = torch.empty(n, dtype=torch.int32, device=dev)
+    o, op = _lib.host_f8(g0._origin)
+    d, dp = _lib.host_f8(g0._gridsize)
+    L = _lib.lib()
+    st = L.fteik3d_raytrace_batch(_ptr(gptr), _ptr(which), _ptr(srcs), len(grids), *g0.shape,
+                                  op, dp, _ptr(pd), n, stepsize, max_step,
+                                  int(bool(honor_grid)), _ptr(rays), _ptr(counts), _ptr(status),
+                                  stream_ptr())
+    _lib.check(st)
+    status = status.cpu().numpy()
+    for code in (_lib.ERR_ARG, _lib.ERR_ENDPOINT_OOB, _lib.ERR_MAX_STEP):
+        if (status == code).any():
+            _lib.check(code)
+    out_all = _ray_lists(rays, counts.cpu().numpy(), n)
+    out, k = [], 0
+    for p, m in zip(pts, sizes):
+        part = out_all[k:k + m]
+        k += m
+        out.append(part[0] if p.ndim == 1 else part)
+    del grads
+    return out
 
 
 class Grid2D(BaseGrid2D):

@@ -96,6 +96,8 @@ SIGNATURES = {
                                  _vp, _vp]),
     "fteik3d_raytrace": (_c_int, [_vp, _c_int, _c_int, _c_int, _dp, _dp, _vp, _c_int, _dp, _c_dbl,
                                   _c_int, _c_int, _vp, _vp, _vp, _vp]),
+    "fteik3d_raytrace_batch": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _dp, _dp,
+                                        _vp, _c_int, _c_dbl, _c_int, _c_int, _vp, _vp, _vp, _vp]),
     "fteik2d_raytrace": (_c_int, [_vp, _c_int, _c_int, _dp, _dp, _vp, _c_int, _dp, _c_dbl, _c_int,
                                   _c_int, _vp, _vp, _vp, _vp]),
 }

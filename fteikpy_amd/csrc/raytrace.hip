@@ -43,24 +43,17 @@ __device__ __forceinline__ void bounds(const Axis &ax, double p, double *lo, dou
     *hi = ax.at(i + 1 < ax.n ? i + 1 : ax.n - 1);
 }
 
+// One ray from the receiver pcur down the gradient grid `grad` to the source srcp: writes
+// its points to ray[0 ..] (receiver first, the source last) and their number minus one to
+// *count; returns 0 or the ray's error code.
 template <int ND>
-__global__ void ray_kernel(const double *grad, Axis A0, Axis A1, Axis A2, const double *p, int n,
-                           double s0, double s1, double s2, double stepsize, int max_step,
-                           int honor_grid, double *rays, int *counts, int *status) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const Axis ax[3] = {A0, A1, A2};
-    const double srcp[3] = {s0, s1, s2};
-    double pcur[3], delta[3], lower[3], upper[3];
-    for (int a = 0; a < ND; ++a) pcur[a] = p[ND * r + a];
-    double *ray = rays + (long long)r * max_step * ND;
-    counts[r] = 0;
-
+__device__ int trace_ray(const double *grad, const Axis (&ax)[3], const double (&srcp)[3],
+                         double (&pcur)[3], double stepsize, int max_step, int honor_grid,
+                         double *ray, int *count_out) {
+    double delta[3], lower[3], upper[3];
+    *count_out = 0;
     for (int a = 0; a < ND; ++a)
-        if (!ax[a].contains(pcur[a])) {
-            status[r] = FTEIK_ERR_ENDPOINT_OOB;
-            return;
-        }
+        if (!ax[a].contains(pcur[a])) return FTEIK_ERR_ENDPOINT_OOB;
     int isrc[3] = {0, 0, 0};
     if (honor_grid) {
         int dummy;
@@ -70,16 +63,16 @@ __global__ void ray_kernel(const double *grad, Axis A0, Axis A1, Axis A2, const 
         }
     }
     auto dist = [&]() {
-        return ND == 3 ? dist3(s0, s1, s2, pcur[0], pcur[1], pcur[2])
-                       : dist2(s0, s1, pcur[0], pcur[1]);
+        return ND == 3 ? dist3(srcp[0], srcp[1], srcp[2], pcur[0], pcur[1], pcur[2])
+                       : dist2(srcp[0], srcp[1], pcur[0], pcur[1]);
     };
     int count = 1;
     for (int a = 0; a < ND; ++a) ray[a] = pcur[a];
     while (dist() >= stepsize) {
         double g[3];
         for (int a = 0; a < ND; ++a)
-            g[a] = ND == 3 ? interp3(A0, A1, A2, grad + a, 3, pcur[0], pcur[1], pcur[2], NAN)
-                           : interp2(A0, A1, grad + a, 2, pcur[0], pcur[1], NAN);
+            g[a] = ND == 3 ? interp3(ax[0], ax[1], ax[2], grad + a, 3, pcur[0], pcur[1], pcur[2], NAN)
+                           : interp2(ax[0], ax[1], grad + a, 2, pcur[0], pcur[1], NAN);
         const double gn = ND == 3 ? sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2])
                                   : sqrt(g[0] * g[0] + g[1] * g[1]);
         double gni;
@@ -114,14 +107,46 @@ __global__ void ray_kernel(const double *grad, Axis A0, Axis A1, Axis A2, const 
             for (int a = 0; a < ND; ++a) ray[ND * count + a] = pcur[a];
             ++count;
         }
-        if (count >= max_step) {
-            status[r] = FTEIK_ERR_MAX_STEP;
-            return;
-        }
+        if (count >= max_step) return FTEIK_ERR_MAX_STEP;
     }
     for (int a = 0; a < ND; ++a) ray[ND * count + a] = srcp[a];
-    counts[r] = count;
-    status[r] = 0;
+    *count_out = count;
+    return 0;
+}
+
+template <int ND>
+__global__ void ray_kernel(const double *grad, Axis A0, Axis A1, Axis A2, const double *p, int n,
+                           double s0, double s1, double s2, double stepsize, int max_step,
+                           int honor_grid, double *rays, int *counts, int *status) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const Axis ax[3] = {A0, A1, A2};
+    const double srcp[3] = {s0, s1, s2};
+    double pcur[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < ND; ++a) pcur[a] = p[ND * r + a];
+    status[r] = trace_ray<ND>(grad, ax, srcp, pcur, stepsize, max_step, honor_grid,
+                              rays + (long long)r * max_step * ND, counts + r);
+}
+
+// Rays of several traveltime grids of one model: ray r follows grid grid_of_ray[r], whose
+// gradient array is grads[k] and source src[3 k ..].
+__global__ void ray_batch_kernel(const double *const *grads, const int *grid_of_ray,
+                                 const double *src, int ngrid, Axis A0, Axis A1, Axis A2,
+                                 const double *p, int n, double stepsize, int max_step,
+                                 int honor_grid, double *rays, int *counts, int *status) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int k = grid_of_ray[r];
+    if (k < 0 || k >= ngrid) {  // not a grid of the batch
+        counts[r] = 0;
+        status[r] = FTEIK_ERR_ARG;
+        return;
+    }
+    const Axis ax[3] = {A0, A1, A2};
+    const double srcp[3] = {src[3 * k], src[3 * k + 1], src[3 * k + 2]};
+    double pcur[3] = {p[3 * r], p[3 * r + 1], p[3 * r + 2]};
+    status[r] = trace_ray<3>(grads[k], ax, srcp, pcur, stepsize, max_step, honor_grid,
+                             rays + (long long)r * max_step * 3, counts + r);
 }
 
 }  // namespace
@@ -141,6 +166,22 @@ extern "C" int fteik3d_raytrace(const double *grad, int n0, int n1, int n2, cons
     hipLaunchKernelGGL(ray_kernel<3>, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream, grad,
                        A0, A1, A2, p, n, src[0], src[1], src[2], stepsize, max_step, honor_grid,
                        rays, counts, status);
+    return hip_status(hipGetLastError());
+}
+
+extern "C" int fteik3d_raytrace_batch(const double *const *grads, const int *grid_of_ray,
+                                      const double *src, int ngrid, int n0, int n1, int n2,
+                                      const double *origin, const double *d, const double *p,
+                                      int n, double stepsize, int max_step, int honor_grid,
+                                      double *rays, int *counts, int *status, void *stream) {
+    if (!grads || !grid_of_ray || !src || ngrid < 1 || !origin || !d || !p || !rays || !counts ||
+        !status || n < 0 || max_step < 2 || !(stepsize > 0.0) || n0 < 2 || n1 < 2 || n2 < 2)
+        return FTEIK_ERR_ARG;
+    if (n == 0) return 0;
+    Axis A0{origin[0], d[0], n0}, A1{origin[1], d[1], n1}, A2{origin[2], d[2], n2};
+    hipLaunchKernelGGL(ray_batch_kernel, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream,
+                       grads, grid_of_ray, src, ngrid, A0, A1, A2, p, n, stepsize, max_step,
+                       honor_grid, rays, counts, status);
     return hip_status(hipGetLastError());
 }
 

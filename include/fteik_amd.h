@@ -362,6 +362,23 @@ int fteik3d_raytrace(const double *grad, int n0, int n1, int n2, const double *o
                      double stepsize, int max_step, int honor_grid, double *rays, int *counts,
                      int *status, void *stream);
 
+/*
+ * No counterpart in the reference, which traces the rays of one traveltime grid per call
+ * (TraveltimeGrid3D.raytrace, fteikpy/_grid.py:221-265): the rays of several traveltime
+ * grids of one model (same n0, n1, n2, origin and d; one grid per source) in one launch,
+ * each ray exactly as fteik3d_raytrace traces it.  grads: device [ngrid] pointers to the
+ * grids' gradient arrays ([n0, n1, n2, 3] each); grid_of_ray: device [n], the grid each
+ * ray follows (a value outside [0, ngrid) gives that ray status 4, the argument error);
+ * src: device [ngrid, 3] ABSOLUTE sources; p, rays, counts, status as fteik3d_raytrace.
+ * For callers with many sources of one model (one wave per 64 rays instead of a
+ * latency-bound launch per source).
+ */
+int fteik3d_raytrace_batch(const double *const *grads, const int *grid_of_ray,
+                           const double *src, int ngrid, int n0, int n1, int n2,
+                           const double *origin, const double *d, const double *p, int n,
+                           double stepsize, int max_step, int honor_grid, double *rays,
+                           int *counts, int *status, void *stream);
+
 /* 2D twin of fteik3d_raytrace, replaces ray2d (fteikpy/_fteik/_ray2d.py:9-153). */
 int fteik2d_raytrace(const double *grad, int n0, int n1, const double *origin, const double *d,
                      const double *p, int n, const double *src, double stepsize, int max_step,

@@ -339,3 +339,19 @@ def test_sweep_launches_per_set():
     assert L.fteik3d_sweep_launches(160, 320, 320, 256, 0, 0) == 1
     for bad in ((0, 4, 4, 1, 0, 2), (4, 4, 4, 0, 0, 2), (4, 4, 4, 1, -1, 2), (4, 4, 4, 1, 0, 3)):
         assert L.fteik3d_sweep_launches(*bad) == 0
+
+
+def test_raytrace_batch_argument_checks():
+    """The batched ray entry point validates its arguments before touching a device."""
+    from fteikpy_amd import _lib
+
+    L = _lib.lib()
+    fake = ctypes.c_void_p(0x1000)
+    o, op = _lib.host_f8(np.zeros(3))
+    d, dp = _lib.host_f8(np.ones(3))
+    args = [fake, fake, fake, 2, 4, 4, 4, op, dp, fake, 3, 1.0, 10, 0, fake, fake, fake, None]
+    assert L.fteik3d_raytrace_batch(*args[:9], fake, 0, *args[11:]) == 0  # nothing to trace
+    for i, bad in ((0, None), (1, None), (2, None), (3, 0), (4, 1), (11, 0.0), (12, 1)):
+        a = list(args)
+        a[i] = bad
+        assert L.fteik3d_raytrace_batch(*a) == _lib.ERR_ARG, i
