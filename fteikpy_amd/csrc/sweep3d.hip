@@ -1098,18 +1098,27 @@ extern "C" size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc) {
 
 namespace {
 
-// Overlapped-schedule brick lists by tile grid, built once per grid shape (the 8 most
-// recent shapes are kept).
+// Overlapped-schedule brick lists by tile grid, built once per grid shape; the most recent
+// shapes are kept, at most 8 of them and 256 MB of host memory together (a list costs 64
+// bytes per tile).
 std::mutex g_table_mu;
 std::vector<std::shared_ptr<const sched::OverlapTable>> g_tables;
+size_t table_bytes(const sched::OverlapTable &t) {
+    return t.task.size() * sizeof(unsigned long long) + t.first.size() * sizeof(int);
+}
 std::shared_ptr<const sched::OverlapTable> overlap_table_for(int nba, int nbb, int nbc) {
     std::lock_guard<std::mutex> lock(g_table_mu);
     for (const auto &t : g_tables)
         if (t->nba == nba && t->nbb == nbb && t->nbc == nbc) return t;
     auto t = std::make_shared<sched::OverlapTable>();
     if (!sched::overlap_table(nba, nbb, nbc, t.get())) return nullptr;
-    if (g_tables.size() >= 8) g_tables.erase(g_tables.begin());
     g_tables.push_back(t);
+    size_t total = 0;
+    for (const auto &u : g_tables) total += table_bytes(*u);
+    while (g_tables.size() > 1 && (g_tables.size() > 8 || total > ((size_t)256 << 20))) {
+        total -= table_bytes(*g_tables.front());
+        g_tables.erase(g_tables.begin());  // callers holding it keep their copy alive
+    }
     return t;
 }
 
