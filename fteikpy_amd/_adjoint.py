@@ -52,11 +52,14 @@ def _solve(slow, gridsize, rel_src, max_sets, logged):
     L = _lib.lib()
     wsb = L.fteik3d_workspace_size(nz, nx, ny, n)
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
-    tile_a, group = autotune.lookup3d((nz, nx, ny), n)
+    tile_a, group, sched = autotune.choice3d((nz, nx, ny), n)
     if not logged:
+        # the library's fastest plain solve: the recorded (or the overlapped) schedule, sets
+        # replayed from a captured graph (launched directly on the legacy default stream)
+        code = {"overlapped": 2, "diagonal": 1}[sched or "overlapped"]
         st = L.fteik3d_solve(_ptr(slow), nz, nx, ny, *gridsize, rp, n, 0, int(max_sets), 0,
-                             tile_a, group, 1, _ptr(tt), None, _ptr(vzero), _ptr(sets), _ptr(ws),
-                             wsb, 0, stream_ptr())
+                             tile_a, group, code, _ptr(tt), None, _ptr(vzero), _ptr(sets),
+                             _ptr(ws), wsb, 1, stream_ptr())
         _lib.check(st)
         return tt, vzero.cpu().numpy(), sets.cpu().numpy(), None, 0
     max_pass = 8 * int(max_sets)
