@@ -54,6 +54,8 @@ SIGNATURES = {
                                _vp, ctypes.c_size_t, _c_int, _vp]),
     "fteik3d_scratch_capacity": (ctypes.c_size_t, []),
     "fteik3d_sweep_launches": (_c_int, [_c_int] * 6),
+    "fteik3d_graph_cache": (_c_int, [_c_int, ctypes.POINTER(ctypes.c_longlong),
+                                     ctypes.POINTER(ctypes.c_longlong)]),
     "fteik3d_solve_noalloc": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _dp,
                                        _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp,
                                        _vp, _vp, _vp]),

@@ -42,6 +42,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstring>
 #include <memory>
 #include <mutex>
 #include <cmath>
@@ -870,6 +871,119 @@ struct Graphs {
     }
 };
 
+// Set graphs of single-model solves kept across calls.  A later solve replays a captured
+// set when its launches would take the same arguments: the key holds everything those
+// derive from (device, caller buffers, workspace, grid, spacing, source count, brick
+// height, group, gradient flag, schedule).  A solve takes its entry out of the cache while
+// it uses it, so two threads never share one, and puts it back when it returns; beyond
+// kGraphCacheMax entries the least recently used is destroyed once its last replay ended.
+struct SetGraphKey {
+    int dev;
+    const double *slow;
+    const double *tt;
+    const uint8_t *sgn;
+    const int *sets_done;
+    const void *ws;
+    int nz, nx, ny, nsrc, tile_a, group, grad, schedule, overlapped;
+    uint64_t dz, dx, dy;  // bit patterns of the spacings
+};
+
+bool same_key(const SetGraphKey &a, const SetGraphKey &b) {
+    return a.dev == b.dev && a.slow == b.slow && a.tt == b.tt && a.sgn == b.sgn &&
+           a.sets_done == b.sets_done && a.ws == b.ws && a.nz == b.nz && a.nx == b.nx &&
+           a.ny == b.ny && a.nsrc == b.nsrc && a.tile_a == b.tile_a && a.group == b.group &&
+           a.grad == b.grad && a.schedule == b.schedule && a.overlapped == b.overlapped &&
+           a.dz == b.dz && a.dx == b.dx && a.dy == b.dy;
+}
+
+struct SetGraph {
+    SetGraphKey key{};
+    Graphs g;
+    hipEvent_t done = nullptr;           // recorded after the last replay of a solve
+    std::vector<hipGraphNode_t> sweep;   // the sweep launches: every kernel node but one
+    bool known = false;                  // exactly one bookkeeping node was found
+    unsigned gy = 0;                     // source groups the sweep launches have now
+    ~SetGraph() {
+        if (done) {
+            (void)hipEventSynchronize(done);
+            (void)hipEventDestroy(done);
+        }
+    }
+};
+
+constexpr size_t kGraphCacheMax = 8;
+std::mutex g_graph_mu;
+// least recently used first; never destroyed at exit (the HIP runtime may be gone by then)
+std::vector<std::unique_ptr<SetGraph>> &g_graphs = *new std::vector<std::unique_ptr<SetGraph>>;
+long long g_graph_hits = 0, g_graph_misses = 0;
+
+std::unique_ptr<SetGraph> graph_checkout(const SetGraphKey &k) {
+    std::lock_guard<std::mutex> lock(g_graph_mu);
+    for (size_t i = g_graphs.size(); i-- > 0;)
+        if (same_key(g_graphs[i]->key, k)) {
+            std::unique_ptr<SetGraph> g = std::move(g_graphs[i]);
+            g_graphs.erase(g_graphs.begin() + (long)i);
+            ++g_graph_hits;
+            return g;
+        }
+    ++g_graph_misses;
+    return nullptr;
+}
+
+void graph_checkin(std::unique_ptr<SetGraph> g) {
+    std::vector<std::unique_ptr<SetGraph>> drop;  // destroyed outside the lock
+    {
+        std::lock_guard<std::mutex> lock(g_graph_mu);
+        for (size_t i = g_graphs.size(); i-- > 0;)
+            if (same_key(g_graphs[i]->key, g->key)) {
+                drop.push_back(std::move(g_graphs[i]));
+                g_graphs.erase(g_graphs.begin() + (long)i);
+            }
+        g_graphs.push_back(std::move(g));
+        while (g_graphs.size() > kGraphCacheMax) {
+            drop.push_back(std::move(g_graphs.front()));
+            g_graphs.erase(g_graphs.begin());
+        }
+    }
+}
+
+// The sweep launches of a captured set: every kernel node except the set's bookkeeping
+// kernel, recognised by its function.  Without exactly one such node the graph is used
+// for this solve only.
+int find_sweep_nodes(SetGraph &s) {
+    size_t n = 0;
+    FTEIK_CHECK(hipGraphGetNodes(s.g.graph, nullptr, &n));
+    std::vector<hipGraphNode_t> nodes(n);
+    FTEIK_CHECK(hipGraphGetNodes(s.g.graph, nodes.data(), &n));
+    int book = 0;
+    s.sweep.clear();
+    for (size_t i = 0; i < n; ++i) {
+        hipGraphNodeType t;
+        FTEIK_CHECK(hipGraphNodeGetType(nodes[i], &t));
+        if (t != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeParams p;
+        FTEIK_CHECK(hipGraphKernelNodeGetParams(nodes[i], &p));
+        if (p.func == reinterpret_cast<void *>(set_bookkeeping_kernel)) ++book;
+        else s.sweep.push_back(nodes[i]);
+    }
+    s.known = book == 1;
+    if (!s.known) s.sweep.clear();
+    return 0;
+}
+
+// Source groups (grid.y) of every sweep launch of a set whose sweep nodes are known.
+int set_sweep_groups(SetGraph &s, unsigned gy) {
+    if (s.gy == gy) return 0;
+    for (hipGraphNode_t node : s.sweep) {
+        hipKernelNodeParams p;
+        FTEIK_CHECK(hipGraphKernelNodeGetParams(node, &p));
+        p.gridDim.y = gy;
+        FTEIK_CHECK(hipGraphExecKernelNodeSetParams(s.g.exec, node, &p));
+    }
+    s.gy = gy;
+    return 0;
+}
+
 template <bool G, int W>
 void launch_diag_t(dim3 grid, hipStream_t st, const SweepArgs &a) {
     hipLaunchKernelGGL((sweep3d_diag_kernel<G, false, W>), grid, dim3(64 * W), 0, st, a);
@@ -1277,19 +1391,54 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
         return hip_status(hipGetLastError());
     };
 
-    // the sweep set captured once, instantiated on first use
-    Graphs gr;
+    // the sweep set captured once, instantiated on first use, or taken from the cache of
+    // earlier solves' sets (the legacy default stream cannot be captured; the persistent
+    // schedule and logged solves are not kept)
+    const bool graphs = use_graph && stream;
+    const bool keep = graphs && !hv && schedule != 0;
+    SetGraphKey key{};
+    std::unique_ptr<SetGraph> gr;
+    if (keep) {
+        int dev = 0;
+        FTEIK_CHECK(hipGetDevice(&dev));
+        key.dev = dev;
+        key.slow = slow;
+        key.tt = tt;
+        key.sgn = want_grad ? sgn : nullptr;
+        key.sets_done = sets_done;
+        key.ws = workspace;
+        key.nz = nz; key.nx = nx; key.ny = ny;
+        key.nsrc = nsrc;
+        key.tile_a = a.tile_a;
+        key.group = group;
+        key.grad = want_grad ? 1 : 0;
+        key.schedule = schedule;
+        key.overlapped = ov ? 1 : 0;
+        std::memcpy(&key.dz, &dz, sizeof dz);
+        std::memcpy(&key.dx, &dx, sizeof dx);
+        std::memcpy(&key.dy, &dy, sizeof dy);
+        gr = graph_checkout(key);
+    }
     auto run_set = [&]() -> int {
-        if (use_graph && stream) {  // the legacy default stream cannot be captured
-            if (!gr.exec) {
+        if (graphs) {
+            if (!gr) {
+                gr.reset(new SetGraph);
+                gr->key = key;
                 FTEIK_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
                 int st = enqueue_set();
-                hipError_t e = hipStreamEndCapture(stream, &gr.graph);
+                hipError_t e = hipStreamEndCapture(stream, &gr->g.graph);
                 if (st) return st;
                 FTEIK_CHECK(e);
-                FTEIK_CHECK(hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0));
+                FTEIK_CHECK(hipGraphInstantiate(&gr->g.exec, gr->g.graph, nullptr, nullptr, 0));
+                gr->gy = gy;
+                if (keep)
+                    if (int st2 = find_sweep_nodes(*gr)) return st2;
+            } else if (gr->gy != gy) {
+                // a kept set still launches the groups its last solve ended with
+                if (gr->done) FTEIK_CHECK(hipEventSynchronize(gr->done));
+                if (int st2 = set_sweep_groups(*gr, gy)) return st2;
             }
-            return hip_status(hipGraphLaunch(gr.exec, stream));
+            return hip_status(hipGraphLaunch(gr->g.exec, stream));
         }
         return enqueue_set();
     };
@@ -1307,10 +1456,21 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
             status = hip_status(e);
             const unsigned need = (unsigned)std::max(1, ceil_div(nact, group));
             if (!status && schedule != 0 && nact > 0 && 2 * need <= gy) {
-                if (gr.exec) status = set_graph_groups(gr.graph, gr.exec, gy, need);
+                if (gr && gr->known) {
+                    status = set_sweep_groups(*gr, need);
+                } else if (gr) {
+                    status = set_graph_groups(gr->g.graph, gr->g.exec, gy, need);
+                    gr->gy = need;
+                }
                 gy = need;
             }
         }
+    }
+    if (!status && keep && gr && gr->known) {
+        // kept for later solves with the same arguments, once this one's replays are queued
+        if (!gr->done) FTEIK_CHECK(hipEventCreateWithFlags(&gr->done, hipEventDisableTiming));
+        FTEIK_CHECK(hipEventRecord(gr->done, stream));
+        graph_checkin(std::move(gr));
     }
     if (!status && schedule == 0) {
         // a dependency wait that gave up leaves a wrong result: report it
@@ -1333,6 +1493,22 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
 }
 
 }  // namespace
+
+extern "C" int fteik3d_graph_cache(int clear, long long *hits, long long *misses) {
+    std::vector<std::unique_ptr<SetGraph>> drop;  // destroyed outside the lock
+    int held = 0;
+    {
+        std::lock_guard<std::mutex> lock(g_graph_mu);
+        if (clear) {
+            for (auto &g : g_graphs) drop.push_back(std::move(g));
+            g_graphs.clear();
+        }
+        if (hits) *hits = g_graph_hits;
+        if (misses) *misses = g_graph_misses;
+        held = (int)g_graphs.size();
+    }
+    return held;
+}
 
 extern "C" int fteik3d_sweep_launches(int nz, int nx, int ny, int nsrc, int tile_a,
                                       int schedule) {

@@ -75,7 +75,11 @@ size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc);
  * vzero     device [nsrc] slowness of each source cell (output)
  * sets_done device [nsrc] sweep sets executed per source (output)
  * workspace device, at least fteik3d_workspace_size(nsrc) bytes
- * use_graph capture one sweep set as a hipGraph and replay it (1) or launch directly (0)
+ * use_graph capture one sweep set as a hipGraph and replay it (1) or launch directly (0);
+ *           a captured set is kept (see fteik3d_graph_cache) and replayed by later calls
+ *           whose launches take the same arguments: same device, slow, tt, sgn, sets_done
+ *           and workspace pointers, grid, spacing, source count, brick height, group,
+ *           gradient flag and schedule (nsweep and max_sets may differ)
  * Checks all sources before enqueuing anything: returns 1 ("source out of bound") with no
  * partial output, as the reference raises (fteikpy/_fteik/_fteik3d.py:402-407).
  * In converge mode (nsweep <= 0) the call synchronizes the stream once per set; the
@@ -91,6 +95,13 @@ int fteik3d_solve(const double *slow, int nz, int nx, int ny, double dz, double 
  * sweep set that fteik3d_solve makes for this grid, source count, brick height (0 = the
  * default) and schedule; 0 for invalid arguments. */
 int fteik3d_sweep_launches(int nz, int nx, int ny, int nsrc, int tile_a, int schedule);
+
+/* No counterpart in the reference (resource control for callers that solve repeatedly):
+ * the set graphs fteik3d_solve keeps for later calls (at most 8, least recently used
+ * dropped).  clear != 0 destroys every kept graph (each after its last replay finished).
+ * Returns the number of graphs kept; hits / misses (either may be NULL) receive the
+ * process's counts of calls that found / did not find a kept graph. */
+int fteik3d_graph_cache(int clear, long long *hits, long long *misses);
 
 /* Bytes of the file-scope device scratch array of fteik3d_solve_noalloc (64 MiB). */
 size_t fteik3d_scratch_capacity(void);

@@ -355,3 +355,14 @@ def test_raytrace_batch_argument_checks():
         a = list(args)
         a[i] = bad
         assert L.fteik3d_raytrace_batch(*a) == _lib.ERR_ARG, i
+
+
+def test_graph_cache_query_without_device():
+    """The kept-graph query takes no device work: nothing kept before any solve."""
+    from fteikpy_amd import _lib
+
+    L = _lib.lib()
+    h, m = ctypes.c_longlong(-1), ctypes.c_longlong(-1)
+    assert L.fteik3d_graph_cache(0, ctypes.byref(h), ctypes.byref(m)) == 0
+    assert h.value >= 0 and m.value >= 0
+    assert L.fteik3d_graph_cache(1, None, None) == 0
