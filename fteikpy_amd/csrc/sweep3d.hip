@@ -199,8 +199,8 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
     const long long plane_nodes = (long long)NX * NY;
     const long long plane_cells = (long long)A.nx * A.ny;
 
-    // The brick's global loads come in two rounds, each issued before any of it is waited
-    // on.  Round 1: the source of each of the W blocks, the first-set test and the guard.
+    // The brick's global loads come in batches, each issued before any of it is waited
+    // on.  Batch 1: the source of each of the W blocks, the first-set test and the guard.
     // Every source runs the first set, so "not the first set" is read from any source's
     // set count (that of source g0) rather than behind the list entry.
     // nact < 0: the caller left the count of sources still changing to this round (the
@@ -220,7 +220,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
     // skipping is allowed from the second set on, and in the first set while the guard holds
     const bool may_skip = has && (!first || guard_clear);
 
-    // Round 2: the skip bytes of this wave's source (lanes 0..7).  A wave with a source
+    // Batch 2: the skip bytes of this wave's source (lanes 0..7).  A wave with a source
     // sweeps unless it may skip and neither the tile nor an upwind neighbour holds a change
     // next to it (pre >= 0: the caller decided).
     {
@@ -246,7 +246,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
     double *tt = A.tt + (long long)src * A.node_stride;
     uint8_t *sg = GRAD ? A.sgn + (long long)src * A.node_stride : nullptr;
 
-    // Round 3, active bricks only, every load issued before any is waited on: the first
+    // Batch 3, active bricks only, every load issued before any is waited on: the first
     // planes for the ring, the planes in flight in registers, this lane's slowness cells.
     // Staging: node entries e = tid + 64 W n (< 81 W; node (eb, ec) of block e / 81),
     // traveltime and change byte, of the blocks whose source sweeps; nodes outside the

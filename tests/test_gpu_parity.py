@@ -4,8 +4,8 @@ test_oracle.py) on the same inputs.
 Bar: traveltimes are bit-identical to the oracle at every nsweep (each directional pass
 is order-invariant and the library is built without FP contraction), and hence within
 1e-15 of the reference's golden vectors; the number of sets to convergence matches;
-gradients, interpolated values and ray points within 1e-12 relative (the reference bar
-of the north star is 1e-9 / 1e-8).
+gradients, interpolated values and ray points within 1e-12 relative (the required bar
+for converged traveltimes is 1e-9, for interpolated values and ray points 1e-8).
 """
 import os
 
