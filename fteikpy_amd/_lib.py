@@ -143,11 +143,11 @@ def check(status):
         raise RuntimeError("workspace too small")
     if status == ERR_UNSUPPORTED:
         raise NotImplementedError("option not supported by this build")
+    if status == ERR_TIMEOUT:
+        raise RuntimeError("persistent sweep: a grid-barrier wait gave up (result invalid)")
     if status == ERR_UNSETTLED:
         raise RuntimeError("adjoint: the upwind stencils do not reproduce the solved times "
                            "(cycle or inconsistent history)")
-    if status == ERR_TIMEOUT:
-        raise RuntimeError("persistent sweep: a dependency wait gave up (result invalid)")
     raise RuntimeError(f"HIP error {-status}")
 
 

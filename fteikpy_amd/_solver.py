@@ -11,7 +11,8 @@ for a batch of sources).  Keyword-only extensions:
 - ``schedule``: "overlapped" (each launch runs the bricks, of any pass, whose inputs are
   final, so a pass starts while the previous one finishes its far side; bricks under 8
   planes run pass by pass), "diagonal" (one launch per brick anti-diagonal and pass) or
-  "persistent" (one launch per sweep set; currently refused); the results are identical;
+  "persistent" (one cooperative launch per sweep set, a grid-wide barrier between the
+  overlapped list's entries); the results are identical;
   None (the default): the autotuner's recorded choice when ``tile_a`` and ``group`` are
   left at 0 too, else "overlapped";
 - ``use_graph``: capture one 3D sweep set as a hipGraph and replay it.

@@ -14,8 +14,9 @@ on the workload:
   20..28 with the fewest steps down a column, 16 planes from 16 sources, else 8.
 
 - the schedule: "overlapped" (each launch runs the bricks, of any pass, whose inputs are
-  final) or "diagonal" (one launch per pass and brick anti-diagonal); both give the same
-  traveltimes bit for bit.
+  final), "diagonal" (one launch per pass and brick anti-diagonal) or "persistent" (one
+  cooperative launch per sweep set, a grid-wide barrier between the overlapped list's
+  entries); all give the same traveltimes bit for bit.
 
 ``tune3d`` times converged solves of the caller's own workload for a list of candidates
 (first the brick heights at a fixed group, then the groups at the best height, then the
@@ -42,7 +43,7 @@ _BATCH = {}
 
 TILES = (8, 16, 32, 48, -1, 0)  # -1: the whole z column; 0: the library default
 GROUPS = (1, 2, 4)
-SCHEDULES = ("overlapped", "diagonal")  # the first is the library default
+SCHEDULES = ("overlapped", "diagonal", "persistent")  # the first is the library default
 BATCH_TILES = (0, 8, 16, 24, 32)  # 0: the library's rule for a batch
 
 

@@ -1,6 +1,6 @@
-// Brick and task indexing of the 3D sweep schedules, shared by the kernels (sweep3d.hip) and
-// by the CPU model of the persistent schedule's dependency protocol
-// (tests/schedmodel/sched_model.cpp), which compiles this header as plain C++.
+// Brick indexing and launch lists of the 3D sweep schedules, shared by the kernels
+// (sweep3d.hip) and by the CPU checks of the overlapped launch list
+// (tests/schedmodel/overlap_dump.cpp), which compile this header as plain C++.
 #pragma once
 
 #include <cstddef>
@@ -79,59 +79,45 @@ FTEIK_HD void brick_tile(int p, int BA, int BB, int BC, int nba, int nbb, int nb
     *tk = pass_sy(p) > 0 ? BC : nbc - 1 - BC;
 }
 
-// Persistent schedule: task t of a set is pass p = t / (ntiles G), then anti-diagonal d,
-// brick i on it, source group g (G groups).  cum_nb[d] = bricks on anti-diagonals < d
-// (ndiag + 1 entries).
-struct Task {
-    int p, d, i, g;
+// Grid-wide barrier of the persistent schedule: arrivals climb a tree of counters, kBarFan
+// participants per counter (workgroups at level 0, counters of the level below above it),
+// each counter on its own 128-byte line after the generation line 0, so that no counter
+// takes more than kBarFan atomic adds per barrier.  The last arrival at a counter resets it
+// and climbs; the last at the root publishes the barrier's generation, which the others
+// poll.
+constexpr int kBarFan = 16;
+constexpr int kBarLine = 32;          // uints per counter line
+constexpr int kBarLevels = 5;
+constexpr int kBarMaxBlocks = 16384;
+constexpr int kBarLines = 1100;       // generation + every level's counters for kBarMaxBlocks
+struct BarTree {
+    int levels;
+    int off[kBarLevels];    // first line of level k
+    int parts[kBarLevels];  // participants of level k
 };
-FTEIK_HD Task decode_task(long long t, int G, int ntiles, const int *cum_nb, int ndiag) {
-    Task k;
-    const long long per_pass = (long long)ntiles * G;
-    k.p = (int)(t / per_pass);
-    const long long r = t - k.p * per_pass;
-    // anti-diagonal d: cum_nb[d] * G <= r < cum_nb[d+1] * G
-    const int rq = (int)(r / G);
-    int lo = 0, hi = ndiag - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (cum_nb[mid] <= rq) lo = mid;
-        else hi = mid - 1;
-    }
-    k.d = lo;
-    const long long j = r - (long long)cum_nb[k.d] * G;
-    k.i = (int)(j / G);
-    k.g = (int)(j - (long long)k.i * G);
-    return k;
+// Tree for `blocks` (1 .. kBarMaxBlocks) workgroups.
+inline BarTree bar_tree(int blocks) {
+    BarTree t{};
+    int parts = blocks, line = 1, k = 0, n;
+    do {
+        n = (parts + kBarFan - 1) / kBarFan;
+        t.off[k] = line;
+        t.parts[k] = parts;
+        line += n;
+        parts = n;
+        ++k;
+    } while (n > 1);
+    t.levels = k;
+    return t;
 }
-
-// Dependency `slot` (0..29) of a task in pass p on memory tile (ti, tj, tk): slots 0..26
-// are the 3x3x3 neighbourhood, which must have finished pass p - 1 (progress >= p);
-// slots 27..29 the upwind face tiles along z, x, y, which must have finished pass p
-// (progress >= p + 1).  Returns the tile index (ii * nbb + jj) * nbc + kk and sets *need,
-// or -1 when the slot has no dependency (outside the grid, or p = 0 for the
-// neighbourhood).
-FTEIK_HD int task_dependency(int slot, int p, int ti, int tj, int tk, int nba, int nbb, int nbc,
-                             int *need) {
-    int ii, jj, kk;
-    *need = 0;
-    if (slot < 27) {
-        if (p == 0) return -1;
-        ii = ti + slot % 3 - 1;
-        jj = tj + (slot / 3) % 3 - 1;
-        kk = tk + slot / 9 - 1;
-        *need = p;
-    } else if (slot < 30) {
-        const int ax = slot - 27;
-        ii = ti - (ax == 0 ? pass_sz(p) : 0);
-        jj = tj - (ax == 1 ? pass_sx(p) : 0);
-        kk = tk - (ax == 2 ? pass_sy(p) : 0);
-        *need = p + 1;
-    } else {
-        return -1;
-    }
-    if (ii < 0 || ii >= nba || jj < 0 || jj >= nbb || kk < 0 || kk >= nbc) return -1;
-    return (ii * nbb + jj) * nbc + kk;
+// Counter of participant idx at level k: its line, how many arrive there, and the
+// participant index it climbs with.
+struct BarCounter {
+    int line, members, next;
+};
+FTEIK_HD BarCounter bar_counter(const BarTree &t, int k, int idx) {
+    const int c = idx / kBarFan;
+    return {t.off[k] + c, imin(kBarFan, t.parts[k] - c * kBarFan), c};
 }
 
 // Overlapped passes: one launch list per sweep set in which every (pass, tile) brick

@@ -28,17 +28,17 @@
 // time, and the same skips apply from the first set on (see guard3d_kernel for the bound
 // on the slowness this needs).
 //
-// Schedules.  "persistent": one launch per sweep set; resident workgroups take (pass,
-// brick, source group) tasks from a queue in a topological order and wait on per-tile
-// progress counters: the 3 upwind face tiles must have finished this pass and the 3x3x3
-// neighbourhood the previous one (which covers every read-after-write and
-// write-after-read between tiles), so passes overlap across tiles.  "diagonal": one launch
-// per brick anti-diagonal and pass, ordered by the stream.
+// Schedules.  "overlapped": one launch list per sweep set in which each launch runs every
+// brick, of any pass, whose inputs are final (schedule.hpp).  "diagonal": one launch per
+// brick anti-diagonal and pass.  Both are ordered by the stream (and replayed from a
+// captured graph).  "persistent": one cooperative launch per set walks the same list with
+// a grid-wide barrier between its entries.
 #include "fteik_amd.h"
 #include "fteik_device.hpp"
 #include "node_update.hpp"
 #include "history.hpp"
 #include "schedule.hpp"
+
 
 #include <algorithm>
 #include <climits>
@@ -60,9 +60,6 @@ constexpr int kDepth = 2;       // further planes in flight in registers (loop u
 // a slot is rewritten at the end of step s for plane s+kPrefetch; its previous plane was
 // last read (as the upwind plane of lane (7,7)) at step s-1, before that step's barrier
 constexpr int kRing = kPrefetch + kSkew + 2;
-// a dependency wait longer than this many seconds gives up and sets the error flag, so
-// every workgroup exits even if a dependency were never met
-constexpr unsigned long long kWaitSeconds = 10;
 
 // LDS slot pitches: 8 mod 16 (doubles, resp. dwords) spreads the lanes' skewed accesses
 // over the banks.
@@ -87,11 +84,6 @@ struct SweepArgs : Stencil3 {
     const int *sets_done;   // sets completed per source (0 during the first set)
     const int *guard;       // 0: the first set may skip (every cell's slowness is large
                             // enough for untouched neighbours to stay untouched)
-    int *progress;          // persistent: passes finished per (source group, tile)
-    int *queue;             // persistent: next task
-    int *error;             // persistent: set when a dependency wait gave up
-    const int *cum_nb;      // persistent: bricks on anti-diagonals < d
-    unsigned long long wait_ticks;  // persistent: wall-clock ticks of the wait limit
     long long node_stride;  // nodes per source grid
     int nz, nx, ny;         // cells
     int NZ, NX, NY;         // nodes
@@ -575,78 +567,110 @@ __global__ void log_mark_kernel(HistHeader *hdr, unsigned long long *seg,
     hdr->npass = P + 1;
 }
 
-// Persistent schedule: one launch per sweep set.  Task t (taken in increasing order) is
-// pass p = t / (tiles * G), then anti-diagonal d, brick on it, source group g (G groups of
-// the sources still changing).  Every dependency of a task has a smaller index, and a
-// workgroup only takes a task while running, so the smallest unfinished task can always
-// proceed.  A dependency wait longer than kWaitSeconds sets *A.error and gives up so that
-// every workgroup still exits.
-template <bool GRAD, int W>
-__global__ __launch_bounds__(64 * W) void sweep3d_persistent_kernel(const SweepArgs A) {
-    __shared__ int s_task, s_abort;
-    const int nact = *A.nlist;
-    const int G = (nact + W - 1) / W;
-    const long long per_pass = (long long)A.ntiles * G;
-    const long long ntasks = 8 * per_pass;
-    const int tid = threadIdx.x;
-    for (;;) {
-        if (tid == 0) s_task = atomicAdd(A.queue, 1);
-        __syncthreads();
-        const long long t = s_task;
-        __syncthreads();  // s_task is rewritten only after every thread has read it
-        if (t >= ntasks) break;
-        const sched::Task k = sched::decode_task(t, G, A.ntiles, A.cum_nb, A.ndiag);
-        const int p = k.p, g = k.g;
-        int BA, BB, BC;
-        decode_brick(k.d, k.i, A.nba, A.nbb, A.nbc, &BA, &BB, &BC);
-        int ti, tj, tk;
-        sched::brick_tile(p, BA, BB, BC, A.nba, A.nbb, A.nbc, &ti, &tj, &tk);
-        int *prog = A.progress + (long long)g * A.ntiles;
+// a grid-barrier wait longer than this gives up (the launch then ends and reports it)
+constexpr unsigned long long kBarWaitSeconds = 5;
 
-        // wait: lane s checks dependency slot s (the 3x3x3 neighbourhood for pass p-1 done,
-        // the upwind face tiles for pass p done)
-        if (tid < 64) {
-            int need = 0;
-            const int idx = sched::task_dependency(tid, p, ti, tj, tk, A.nba, A.nbb, A.nbc, &need);
-            bool ok = idx < 0;
-            const unsigned long long t_start = wall_clock64();
-            while (!__all(ok)) {
-                if (!ok)
-                    ok = __hip_atomic_load(prog + idx, __ATOMIC_ACQUIRE,
-                                           __HIP_MEMORY_SCOPE_AGENT) >= need;
-                if (__all(ok)) break;
-                // a wait that gave up anywhere ends every wait of the launch: the result is
-                // invalid already, and the launch drains without further time-outs
-                if (__hip_atomic_load(A.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
-                    break;
-                if (wall_clock64() - t_start > A.wait_ticks) {
-                    if (tid == 0) atomicExch(A.error, 1);
+// Grid-wide barrier of the persistent schedule's cooperative launch (tree layout and
+// counter choice in schedule.hpp, shared with its CPU model).  Every thread fences its
+// writes before arriving and after leaving (release / acquire at agent scope).
+// A wait that outlasts wait_ticks (wall clock) sets the give-up flag (word 1 of line 0),
+// which every waiting workgroup also polls: the launch then ends (every workgroup leaves its
+// loop), and the host reports the set's result as invalid.  The cooperative launch keeps
+// every workgroup resident, so a wait ends normally within microseconds.
+struct TreeBar {
+    unsigned *mem;       // line 0: generation, give-up flag; then the counters, level by level
+    sched::BarTree t;
+    unsigned long long wait_ticks;
+};
+
+// Returns false once any wait of the launch gave up.
+__device__ bool tree_barrier(const TreeBar &B, unsigned target) {
+    __shared__ int s_ok;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int idx = (int)blockIdx.x;
+        bool last = true;
+        for (int k = 0; k < B.t.levels && last; ++k) {
+            const sched::BarCounter c = sched::bar_counter(B.t, k, idx);
+            unsigned *ctr = B.mem + (size_t)c.line * sched::kBarLine;
+            const unsigned old =
+                __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            last = old == (unsigned)(c.members - 1);
+            if (last) __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            idx = c.next;
+        }
+        unsigned *flag = B.mem + 1;
+        if (last) {
+            __hip_atomic_store(B.mem, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const unsigned long long t0 = wall_clock64();
+            while (__hip_atomic_load(B.mem, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != target) {
+                if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+                if (wall_clock64() - t0 > B.wait_ticks) {
+                    __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
-                __builtin_amdgcn_s_sleep(2);
+                __builtin_amdgcn_s_sleep(1);
             }
-            if (tid == 0)
-                s_abort = __hip_atomic_load(A.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __syncthreads();
-        if (!s_abort) {
-            __threadfence();  // acquire for every wave: see the producers' data
-            brick<GRAD, false, W>(A, p, BA, BB, BC, g * W, nact);
-            __threadfence();  // release this thread's writes
-        }
-        __syncthreads();
-        if (tid == 0)
-            __hip_atomic_store(prog + (ti * A.nbb + tj) * A.nbc + tk, p + 1, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
+        s_ok = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
     }
+    __syncthreads();
+    __threadfence();
+    return s_ok != 0;
 }
 
-// Start of every set on the persistent schedule: reset queue and progress counters.
-__global__ void set_reset_kernel(int *queue, int *progress, long long n) {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i0 == 0) *queue = 0;
-    for (long long i = i0; i < n; i += stride) progress[i] = 0;
+// Persistent schedule: one cooperative launch per sweep set, every workgroup resident (the
+// runtime refuses the launch otherwise).  The launch walks the set's launch list -- the
+// overlapped list (first[l] .. first[l+1] bricks of tasks) or, without one, one entry per
+// (pass, brick anti-diagonal) -- and between entries waits at a grid-wide barrier, which
+// orders every write of one entry before every read of the next as a launch boundary does.
+// Within an entry, workgroups take (brick, source group) items from the entry's counter
+// (zeroed before the launch), so a workgroup whose bricks are skipped takes the next.
+// Every workgroup runs every entry's loop and barrier: no early exit.
+// No register limit that would spill: a kernel using scratch can have fewer waves resident
+// than the occupancy query reports (the runtime's scratch slots bound them), and a
+// workgroup left waiting for a slot would never reach the grid barrier (the host refuses
+// the schedule for a build whose kernel uses scratch).
+template <bool GRAD, int W>
+__global__ __launch_bounds__(64 * W)
+void sweep3d_coop_kernel(const SweepArgs A, const int *first, int nentry, unsigned gy,
+                         unsigned *ticket, TreeBar bar) {
+    __shared__ unsigned s_item, s_base;
+    if (threadIdx.x == 0)
+        s_base = __hip_atomic_load(bar.mem, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const unsigned base = s_base;  // read by every workgroup before any arrives
+    for (int l = 0; l < nentry; ++l) {
+        int nb, f0 = 0, pass = 0, d = 0;
+        if (first) {
+            f0 = first[l];
+            nb = first[l + 1] - f0;
+        } else {
+            pass = l / A.ndiag;
+            d = l - pass * A.ndiag;
+            nb = sched::bricks_on_diag(d, A.nba, A.nbb, A.nbc);
+        }
+        const unsigned nitems = (unsigned)nb * gy;
+        // (an item is taken only when the previous one is done: taking the next one ahead
+        // leaves it waiting behind a running brick at the end of an entry)
+        for (;;) {
+            if (threadIdx.x == 0) s_item = atomicAdd(ticket + l, 1u);
+            __syncthreads();
+            const unsigned i = s_item;
+            __syncthreads();  // s_item is rewritten only after every thread has read it
+            if (i >= nitems) break;
+            const int x = (int)(i % (unsigned)nb), y = (int)(i / (unsigned)nb);
+            int p = pass, BA, BB, BC;
+            if (first)
+                sched::unpack_task(A.tasks[f0 + x], &p, &BA, &BB, &BC);
+            else
+                decode_brick(d, x, A.nba, A.nbb, A.nbc, &BA, &BB, &BC);
+            brick<GRAD, false, W>(A, p, BA, BB, BC, y * W, -1);
+        }
+        if (!tree_barrier(bar, base + (unsigned)l + 1u)) break;  // uniform in the workgroup
+    }
 }
 
 // tt = Big everywhere (and signs cleared), _fteik3d.py:429-437
@@ -994,9 +1018,19 @@ void launch_diag_log_t(dim3 grid, hipStream_t st, const SweepArgs &a) {
     hipLaunchKernelGGL((sweep3d_diag_kernel<false, true, W>), grid, dim3(64 * W), 0, st, a);
 }
 
+// Persistent schedule: the set's cooperative launch (error code of the launch call).
 template <bool G, int W>
-void launch_pers_t(int blocks, hipStream_t st, const SweepArgs &a) {
-    hipLaunchKernelGGL((sweep3d_persistent_kernel<G, W>), dim3(blocks), dim3(64 * W), 0, st, a);
+void launch_coop_t(int blocks, hipStream_t st, const SweepArgs &a, const int *first, int nentry,
+                   unsigned gy, unsigned *ticket, const TreeBar &bar, hipError_t *err) {
+    SweepArgs args = a;
+    const int *f = first;
+    int n = nentry;
+    unsigned g = gy;
+    unsigned *t = ticket;
+    TreeBar b = bar;
+    void *params[] = {&args, &f, &n, &g, &t, &b};
+    *err = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(sweep3d_coop_kernel<G, W>),
+                                      dim3(blocks), dim3(64 * W), params, 0, st);
 }
 
 #define FTEIK_DISPATCH_W(FN, grad, group, ...)                          \
@@ -1011,16 +1045,27 @@ void launch_pers_t(int blocks, hipStream_t st, const SweepArgs &a) {
         default: FN<true, 8>(__VA_ARGS__); break;                       \
     }
 
+// Workgroups of the persistent schedule's launch: as many as can be resident at once.
 template <bool G, int W>
 int pers_blocks_t() {
-    int dev = 0, cus = 0, per = 0;
+    int dev = 0, cus = 0, per = 0, coop = 0;
     if (hipGetDevice(&dev) != hipSuccess) return -1;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
+        !coop)
+        return -1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return -1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, sweep3d_persistent_kernel<G, W>,
-                                                     64 * W, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, sweep3d_coop_kernel<G, W>, 64 * W,
+                                                     0) != hipSuccess ||
+        per < 1)
         return -1;
-    return cus * std::max(per, 1);
+    // every workgroup must be resident at once: refuse a kernel that needs scratch memory
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(sweep3d_coop_kernel<G, W>)) !=
+            hipSuccess ||
+        fa.localSizeBytes != 0)
+        return -1;
+    return cus * per;
 }
 
 int pers_blocks(bool grad, int group) {
@@ -1038,7 +1083,10 @@ int pers_blocks(bool grad, int group) {
 
 struct Ws3 {
     SrcParam *prm;
-    int *changed, *active, *list, *nlist, *queue, *error, *guard, *cum_nb, *progress;
+    int *changed, *active, *list, *nlist, *guard;
+    int *first;         // persistent schedule: launch list boundaries (overlapped list)
+    unsigned *ticket;   // persistent schedule: item counter per launch list entry
+    unsigned *bar;      // persistent schedule: grid barrier lines (sched::kBarLines lines of 128 bytes)
     uint8_t *chg, *tsum;
     uint64_t *faces;
     unsigned long long *tasks;
@@ -1072,9 +1120,12 @@ size_t ws_bytes(int nz, int nx, int ny, int nsrc, Ws3 *w, char *base) {
     char *changed = take(sizeof(int) * nsrc);
     char *active = take(sizeof(int) * nsrc);
     char *list = take(sizeof(int) * nsrc);
-    char *misc = take(sizeof(int) * 8);  // nlist, queue, error, guard
-    char *cum = take(sizeof(int) * (max_diag(nz, nx, ny) + 1));
-    char *prog = take(sizeof(int) * tiles * nsrc);
+    char *misc = take(sizeof(int) * 8);  // nlist, guard
+    // a set's launch list has at most 8 entries per anti-diagonal of the tile grid
+    const size_t entries = 8 * max_diag(nz, nx, ny);
+    char *first = take(sizeof(int) * (entries + 1));
+    char *ticket = take(sizeof(unsigned) * entries);
+    char *bar = take(sizeof(unsigned) * sched::kBarLine * sched::kBarLines);
     char *chg = take(nodes * nsrc);
     char *tsum = take(tiles * nsrc);
     char *faces = take(sizeof(uint64_t) * face_tiles(nz, nx, ny) * nsrc);
@@ -1085,11 +1136,10 @@ size_t ws_bytes(int nz, int nx, int ny, int nsrc, Ws3 *w, char *base) {
         w->active = (int *)active;
         w->list = (int *)list;
         w->nlist = (int *)misc;
-        w->queue = (int *)misc + 1;
-        w->error = (int *)misc + 2;
-        w->guard = (int *)misc + 3;
-        w->cum_nb = (int *)cum;
-        w->progress = (int *)prog;
+        w->guard = (int *)misc + 1;
+        w->first = (int *)first;
+        w->ticket = (unsigned *)ticket;
+        w->bar = (unsigned *)bar;
         w->chg = (uint8_t *)chg;
         w->tsum = (uint8_t *)tsum;
         w->faces = (uint64_t *)faces;
@@ -1169,10 +1219,6 @@ void make_args(const double *slow, int nz, int nx, int ny, double dz, double dx,
     a.nlist = w.nlist;
     a.sets_done = sets_done;
     a.guard = w.guard;
-    a.progress = w.progress;
-    a.queue = w.queue;
-    a.error = w.error;
-    a.cum_nb = w.cum_nb;
     a.node_stride = nn;
     a.nz = nz; a.nx = nx; a.ny = ny;
     a.NZ = NZ; a.NX = NX; a.NY = NY;
@@ -1191,7 +1237,6 @@ void make_args(const double *slow, int nz, int nx, int ny, double dz, double dx,
     a.log_old = nullptr;
     a.log_count = nullptr;
     a.log_cap = 0;
-    a.wait_ticks = 0;
     a.gmax = ceil_div(nsrc, group);
     if (a.tile_a >= kFaceMinTile) a.faces = w.faces;
     static_cast<Stencil3 &>(a) = make_stencil3(dz, dx, dy);
@@ -1267,10 +1312,6 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
     if (group == 0) group = nsrc >= 8 ? 2 : 1;
     if (group != 1 && group != 2 && group != 4 && group != 8) return FTEIK_ERR_ARG;
     if (schedule < 0 || schedule > 2) return FTEIK_ERR_ARG;
-    // The persistent schedule stalled on its first GPU run (dependency waits on per-tile
-    // progress counters were not satisfied) and its cause is not established: it is
-    // switched off and refused until it is.
-    if (schedule == 0) return FTEIK_ERR_UNSUPPORTED;
     hipStream_t stream = (hipStream_t)stream_;
 
     std::vector<SrcParam> prm(nsrc);
@@ -1291,34 +1332,54 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
     a.log_cap = hv ? hcap : 0;
     std::vector<int> nb(a.ndiag);
     for (int d = 0; d < a.ndiag; ++d) nb[d] = bricks_on_diag(d, a.nba, a.nbb, a.nbc);
-    std::vector<int> cum(a.ndiag + 1, 0);
-    for (int d = 0; d < a.ndiag; ++d) cum[d + 1] = cum[d] + nb[d];
     const long long gmax = a.gmax;
-    if (8LL * a.ntiles * gmax >= (1LL << 31)) return FTEIK_ERR_ARG;  // task index is an int
+    // (brick, source group) items of a launch are counted in 32 bits
+    if (8LL * a.ntiles * gmax >= (1LL << 31)) return FTEIK_ERR_ARG;
     // overlapped passes: bricks of at least kOverlapMinTile planes, and no history (whose
     // segments are closed pass by pass); otherwise one pass at a time
     std::shared_ptr<const sched::OverlapTable> ov;
-    if (schedule == 2 && !hv && a.tile_a >= kOverlapMinTile) {
+    if (schedule != 1 && !hv && a.tile_a >= kOverlapMinTile) {
         ov = overlap_table_for(a.nba, a.nbb, a.nbc);
         if (!ov || ov->task.size() > 8 * face_tiles(nz, nx, ny)) return FTEIK_ERR_ARG;
         a.tasks = w.tasks;
     }
+    // persistent schedule: one cooperative launch per set over nentry launch-list entries
     const int pblocks = schedule == 0 ? pers_blocks(want_grad != 0, group) : 0;
-    if (schedule == 0 && pblocks <= 0) return FTEIK_ERR_ARG;
-    {
+    if (schedule == 0 && (pblocks <= 0 || hv)) return FTEIK_ERR_UNSUPPORTED;
+    const int nentry = ov ? ov->launches() : 8 * a.ndiag;
+    if (schedule == 0 && (long long)nentry > 8LL * (long long)max_diag(nz, nx, ny))
+        return FTEIK_ERR_ARG;
+    TreeBar bar{};
+    bar.mem = w.bar;
+    int coop_blocks = 0;
+    if (schedule == 0) {
+        // no more workgroups than the largest entry has (brick, source group) items
+        long long most = 0;
+        if (ov) {
+            for (int l = 0; l < nentry; ++l)
+                most = std::max<long long>(most, ov->first[l + 1] - ov->first[l]);
+        } else {
+            for (int d = 0; d < a.ndiag; ++d) most = std::max<long long>(most, nb[d]);
+        }
+        most *= gmax;
+        coop_blocks = (int)std::min<long long>(
+            {(long long)pblocks, most, (long long)sched::kBarMaxBlocks});
+        coop_blocks = std::max(coop_blocks, 1);
+        bar.t = sched::bar_tree(coop_blocks);
         int dev = 0, khz = 0;
         FTEIK_CHECK(hipGetDevice(&dev));
         FTEIK_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-        a.wait_ticks = (unsigned long long)std::max(khz, 1) * 1000ull * kWaitSeconds;
+        bar.wait_ticks = (unsigned long long)std::max(khz, 1) * 1000ull * kBarWaitSeconds;
     }
 
     FTEIK_CHECK(hipMemcpyAsync(w.prm, prm.data(), sizeof(SrcParam) * nsrc, hipMemcpyHostToDevice,
                                stream));
-    FTEIK_CHECK(hipMemcpyAsync(w.cum_nb, cum.data(), sizeof(int) * (a.ndiag + 1),
-                               hipMemcpyHostToDevice, stream));
     if (ov)
         FTEIK_CHECK(hipMemcpyAsync(w.tasks, ov->task.data(),
                                    sizeof(unsigned long long) * ov->task.size(),
+                                   hipMemcpyHostToDevice, stream));
+    if (ov && schedule == 0)
+        FTEIK_CHECK(hipMemcpyAsync(w.first, ov->first.data(), sizeof(int) * ov->first.size(),
                                    hipMemcpyHostToDevice, stream));
     // the host vectors go out of scope on return: finish the copies first
     FTEIK_CHECK(hipStreamSynchronize(stream));
@@ -1340,9 +1401,12 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
             FTEIK_CHECK(hipGetLastError());
         }
         if (int st = launch_seed3d(a, w, slow, nsrc, vzero, sets_done, stream)) return st;
-        hipLaunchKernelGGL(zero_bytes_kernel, dim3(1), dim3(64), 0, stream, (uint8_t *)w.error,
-                           (long long)sizeof(int));
-        FTEIK_CHECK(hipGetLastError());
+        if (schedule == 0) {  // the barrier's counters start at zero (and end each barrier so)
+            const long long nbar = (long long)sizeof(unsigned) * sched::kBarLine * sched::kBarLines;
+            hipLaunchKernelGGL(zero_bytes_kernel, dim3((unsigned)((nbar + 255) / 256)), dim3(256), 0,
+                               stream, (uint8_t *)w.bar, nbar);
+            FTEIK_CHECK(hipGetLastError());
+        }
         if (hv) {
             const int max_pass = 8 * (nsweep > 0 ? nsweep : max_sets);
             hipLaunchKernelGGL(log_init_kernel, dim3(1), dim3(1), 0, stream, hv->hdr, hv->seg,
@@ -1357,11 +1421,14 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
     unsigned gy = (unsigned)gmax;
     auto enqueue_set = [&]() -> int {
         if (schedule == 0) {
-            const long long np = (long long)a.ntiles * gmax;
-            hipLaunchKernelGGL(set_reset_kernel,
-                               dim3((unsigned)std::min<long long>((np + 255) / 256, 4096)), dim3(256),
-                               0, stream, w.queue, w.progress, np);
-            FTEIK_DISPATCH_W(launch_pers_t, want_grad != 0, group, pblocks, stream, a);
+            const long long nt = (long long)sizeof(unsigned) * nentry;
+            hipLaunchKernelGGL(zero_bytes_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0,
+                               stream, (uint8_t *)w.ticket, nt);
+            FTEIK_CHECK(hipGetLastError());
+            hipError_t err = hipSuccess;
+            FTEIK_DISPATCH_W(launch_coop_t, want_grad != 0, group, coop_blocks, stream, a,
+                             ov ? w.first : nullptr, nentry, gy, w.ticket, bar, &err);
+            FTEIK_CHECK(err);
         } else if (ov) {
             for (int l = 0; l < ov->launches(); ++l) {
                 a.diag = ov->first[l];
@@ -1394,7 +1461,7 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
     // the sweep set captured once, instantiated on first use, or taken from the cache of
     // earlier solves' sets (the legacy default stream cannot be captured; the persistent
     // schedule and logged solves are not kept)
-    const bool graphs = use_graph && stream;
+    const bool graphs = use_graph && stream && schedule != 0;  // cooperative launches run direct
     const bool keep = graphs && !hv && schedule != 0;
     SetGraphKey key{};
     std::unique_ptr<SetGraph> gr;
@@ -1451,11 +1518,17 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
         for (int s = 0; s < max_sets && nact > 0 && !status; ++s) {
             status = run_set();
             if (status) break;
+            unsigned gave_up = 0;
             hipError_t e = hipMemcpyAsync(&nact, w.nlist, sizeof(int), hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess && schedule == 0)
+                e = hipMemcpyAsync(&gave_up, w.bar + 1, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                   stream);
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
             status = hip_status(e);
+            if (!status && gave_up) status = FTEIK_ERR_TIMEOUT;
+            if (status) break;
             const unsigned need = (unsigned)std::max(1, ceil_div(nact, group));
-            if (!status && schedule != 0 && nact > 0 && 2 * need <= gy) {
+            if (!status && nact > 0 && 2 * need <= gy) {
                 if (gr && gr->known) {
                     status = set_sweep_groups(*gr, need);
                 } else if (gr) {
@@ -1466,19 +1539,20 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
             }
         }
     }
+    if (!status && schedule == 0 && nsweep > 0) {
+        // a grid-barrier wait that gave up leaves an invalid result: report it
+        unsigned gave_up = 0;
+        hipError_t e = hipMemcpyAsync(&gave_up, w.bar + 1, sizeof(unsigned),
+                                      hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        status = hip_status(e);
+        if (!status && gave_up) status = FTEIK_ERR_TIMEOUT;
+    }
     if (!status && keep && gr && gr->known) {
         // kept for later solves with the same arguments, once this one's replays are queued
         if (!gr->done) FTEIK_CHECK(hipEventCreateWithFlags(&gr->done, hipEventDisableTiming));
         FTEIK_CHECK(hipEventRecord(gr->done, stream));
         graph_checkin(std::move(gr));
-    }
-    if (!status && schedule == 0) {
-        // a dependency wait that gave up leaves a wrong result: report it
-        int err = 0;
-        hipError_t e = hipMemcpyAsync(&err, w.error, sizeof(int), hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        status = hip_status(e);
-        if (!status && err) status = FTEIK_ERR_TIMEOUT;
     }
     if (!status && hv) {
         unsigned long long count = 0;

@@ -31,7 +31,7 @@ extern "C" {
 #define FTEIK_ERR_MAX_STEP 3
 #define FTEIK_ERR_ARG 4
 #define FTEIK_ERR_WORKSPACE 5
-#define FTEIK_ERR_TIMEOUT 6
+#define FTEIK_ERR_TIMEOUT 6 /* persistent schedule: a grid-barrier wait gave up */
 #define FTEIK_ERR_UNSUPPORTED 7
 #define FTEIK_ERR_UNSETTLED 8
 
@@ -67,9 +67,13 @@ size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc);
  *           in earlier launches (508 launches per set instead of 688 for 160x320x320
  *           cells in 27-plane bricks); bricks under 8 planes run as schedule 1;
  *           1: one launch per (pass, brick anti-diagonal), ordered by the stream;
- *           0: persistent (one launch per sweep set, tasks wait on per-tile progress
- *           counters) -- currently refused with FTEIK_ERR_UNSUPPORTED;
- *           1 and 2 give identical results
+ *           0: persistent: one cooperative launch per sweep set (every workgroup
+ *           resident) walking schedule 2's launch list (schedule 1's for bricks under 8
+ *           planes) with a grid-wide barrier between entries; FTEIK_ERR_UNSUPPORTED on a
+ *           device without cooperative launches, FTEIK_ERR_TIMEOUT (result invalid) if a
+ *           barrier wait outlasted 5 s; use_graph is ignored; the call synchronizes the
+ *           stream at the end (fixed nsweep) to read the barrier's give-up flag;
+ *           0, 1 and 2 give identical results
  * tt        device [nsrc, nz+1, nx+1, ny+1] traveltimes (output)
  * sgn       device [nsrc, nz+1, nx+1, ny+1] uint8 packed upwind signs or NULL
  * vzero     device [nsrc] slowness of each source cell (output)

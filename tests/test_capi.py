@@ -115,8 +115,9 @@ def test_product_does_not_import_oracle():
                 assert "libfteik_oracle" not in text, f
 
 
-def test_persistent_schedule_is_refused():
-    """The persistent schedule is switched off: the call returns before any device work."""
+def test_schedule_argument_checks():
+    """Unknown schedules are refused, and the persistent schedule without a device able to
+    run cooperative launches (none here), before any device work."""
     from fteikpy_amd import _lib
 
     L = _lib.lib()
@@ -255,8 +256,10 @@ def test_autotune_records_and_persists(tmp_path):
     assert autotune.choice3d((10, 20, 30), 4) == (16, 2, None)
     autotune.record3d((10, 20, 30), 8, 24, 1, "diagonal")
     assert autotune.choice3d((10, 20, 30), 8) == (24, 1, "diagonal")
+    autotune.record3d((10, 20, 30), 8, 24, 1, "persistent")
+    assert autotune.choice3d((10, 20, 30), 8) == (24, 1, "persistent")
     with pytest.raises(ValueError):
-        autotune.record3d((10, 20, 30), 8, 24, 1, "persistent")
+        autotune.record3d((10, 20, 30), 8, 24, 1, "bogus")
 
     class _Eik:  # stand-ins: the batch key needs each problem's shape and source count
         def __init__(self, shape):

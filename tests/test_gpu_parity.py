@@ -96,18 +96,13 @@ def _rough_model(shape, seed):
     return 1500.0 * np.exp(0.8 * p / np.abs(p).max())
 
 
-PERSISTENT_OFF = pytest.mark.skip(
-    reason="persistent schedule stalled on its first GPU run; cause not established, "
-           "switched off in the library (FTEIK_ERR_UNSUPPORTED)")
-
-
 @pytest.mark.parametrize("tile_a, group, schedule", [
     (1, 1, "diagonal"), (3, 2, "diagonal"), (8, 4, "diagonal"), (0, 0, "diagonal"),
     (64, 8, "diagonal"), (5, 4, "diagonal"),
     (8, 2, "overlapped"), (9, 1, "overlapped"), (0, 0, "overlapped"), (64, 8, "overlapped"),
     (10, 4, "overlapped"), (3, 2, "overlapped"),
-    pytest.param(3, 2, "persistent", marks=PERSISTENT_OFF),
-    pytest.param(0, 4, "persistent", marks=PERSISTENT_OFF)])
+    (3, 2, "persistent"), (0, 4, "persistent"), (1, 1, "persistent"), (9, 1, "persistent"),
+    (64, 8, "persistent")])
 def test_brick_shapes_bitwise(fa, tile_a, group, schedule):
     """Partial bricks in every axis, several z brick heights and source groups (groups
     partly empty), sources converging after different numbers of sets."""
@@ -138,9 +133,10 @@ def test_graph_mode_matches_direct(fa):
 
 
 def test_overlapped_equals_pass_by_pass(fa):
-    """The overlapped schedule (bricks of several passes in one launch) and one pass per
-    launch give identical traveltimes, upwind signs and set counts at every nsweep and
-    converged, directly and from a captured graph, and both equal the oracle."""
+    """The overlapped schedule (bricks of several passes in one launch), one pass per
+    launch and the persistent schedule (one cooperative launch per set) give identical
+    traveltimes, upwind signs and set counts at every nsweep and converged, directly and
+    from a captured graph, and equal the oracle."""
     shape, d = (40, 45, 38), (9.0, 10.0, 11.0)
     v = _rough_model(shape, 23)
     rng = np.random.default_rng(23)
@@ -152,9 +148,12 @@ def test_overlapped_equals_pass_by_pass(fa):
                                   group=group, schedule="diagonal")
             b = eik.solve_tensors(srcs, nsweep=ns, return_gradient=True, tile_a=tile,
                                   group=group, schedule="overlapped", use_graph=graph)
-            assert np.array_equal(a[0].cpu().numpy(), b[0].cpu().numpy()), (ns, tile)
-            assert np.array_equal(a[1].cpu().numpy(), b[1].cpu().numpy()), (ns, tile)
-            assert list(a[3]) == list(b[3]), (ns, tile)
+            c = eik.solve_tensors(srcs, nsweep=ns, return_gradient=True, tile_a=tile,
+                                  group=group, schedule="persistent")
+            for x in (b, c):
+                assert np.array_equal(a[0].cpu().numpy(), x[0].cpu().numpy()), (ns, tile)
+                assert np.array_equal(a[1].cpu().numpy(), x[1].cpu().numpy()), (ns, tile)
+                assert list(a[3]) == list(x[3]), (ns, tile)
     ref, _, _, kref = oracle.solve3d(1.0 / v, *d, srcs, nsweep=0)
     assert list(b[3]) == list(kref)
     assert np.array_equal(b[0].cpu().numpy(), ref)
