@@ -256,8 +256,8 @@ def test_autotune_records_and_persists(tmp_path):
     assert autotune.choice3d((10, 20, 30), 4) == (16, 2, None)
     autotune.record3d((10, 20, 30), 8, 24, 1, "diagonal")
     assert autotune.choice3d((10, 20, 30), 8) == (24, 1, "diagonal")
-    autotune.record3d((10, 20, 30), 8, 24, 1, "persistent")
-    assert autotune.choice3d((10, 20, 30), 8) == (24, 1, "persistent")
+    autotune.record3d((10, 20, 30), 9, 24, 1, "persistent")
+    assert autotune.choice3d((10, 20, 30), 9) == (24, 1, "persistent")
     with pytest.raises(ValueError):
         autotune.record3d((10, 20, 30), 8, 24, 1, "bogus")
 
