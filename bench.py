@@ -85,6 +85,11 @@ def main():
     ap.add_argument("--profile-2d", action="store_true",
                     help="only run the 2D workloads (the plain 2D case at nsweep 2, the 2D "
                          "models of the uneven batch) and print their times, for profiling")
+    ap.add_argument("--profile-schedules", action="store_true",
+                    help="only run converged solves of the single-source 200^3 workload and the "
+                         "batched workload (brick height and group at the library defaults) on "
+                         "the overlapped and the persistent schedule, a warm-up and one timed "
+                         "solve each, and print their times, for profiling")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only copy GIB GiB once on the device (torch clone) and exit: the "
                          "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -102,6 +107,9 @@ def main():
 
     if args.profile_2d:
         profile_2d()
+        return
+    if args.profile_schedules:
+        profile_schedules(args)
         return
     if args.profile_batch or args.profile_plain or args.profile_post:
         profile_sections(args)
@@ -516,6 +524,48 @@ def profile_sections(args):
                 dt = time.perf_counter() - t0
                 out = {"plain_sources": n, "seconds": dt, "source_grids_per_s": n / dt,
                        "sets": sets}
+    print(json.dumps(out), flush=True)
+
+
+def profile_schedules(args):
+    """Overlapped (graph-replayed launches) against persistent (one cooperative launch per
+    sweep set) on the single-source 200^3 workload and the batched workload: a warm-up and
+    one timed converged solve each, on one stream; prints the times and whether the
+    persistent solves equal the overlapped ones bit for bit."""
+    import torch
+
+    import fteikpy_amd as fa
+    from fteikpy_amd import synthetic
+
+    dev = torch.device("cuda", 0)
+    shape = tuple(int(x) for x in args.shape.split(","))
+    cases = [("single_source_200cubed", synthetic.model_3d_layers()),
+             ("batched", synthetic.model_3d_inclusions(shape=shape))]
+    out = {}
+    with torch.cuda.stream(torch.cuda.Stream(device=dev)):
+        for name, (v, d, src, _) in cases:
+            eik = fa.Eikonal3D(v, d)
+            res = {}
+            ref = None
+            for sched in ("overlapped", "persistent"):
+                def solve():
+                    return eik.solve_tensors(src, nsweep=None, tile_a=0, group=0, schedule=sched,
+                                             use_graph=sched == "overlapped")
+                solve()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                tt, _, _, sets = solve()
+                torch.cuda.synchronize()
+                res[sched] = time.perf_counter() - t0
+                log(f"{name} {sched}: {res[sched]:.4f} s")
+                if ref is None:
+                    ref = (tt, list(sets))
+                else:
+                    res["persistent_equals_overlapped_bitwise"] = bool(
+                        torch.equal(tt, ref[0]) and list(sets) == ref[1])
+                    del tt
+            del ref
+            out[name] = res
     print(json.dumps(out), flush=True)
 
 
