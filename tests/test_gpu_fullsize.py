@@ -2,7 +2,8 @@
 
 - Many sources on one model (160 x 320 x 320, 12.5 m, 20 inclusions, 256 surface
   sources): all 256 solved at nsweep 1 and 2, four of them (corners and centre of the source
-  grid) bit-identical to the oracle, with their 4096 receivers; for every source of the
+  grid) bit-identical to the oracle, with their 4096 receivers, and all 256 bit-identical
+  on the persistent schedule; for every source of the
   converged batch, the grid one set before convergence already final (the last set
   changed nothing) and, for the four, two sets before not yet.
 - Uneven batch of 48 models (16 2D up to 1024 per side, 32 3D up to 192 x 256 x 256, 765
@@ -60,11 +61,17 @@ def inclusions(fa):
 @pytest.mark.parametrize("nsweep", [1, 2])
 def test_many_sources_first_sets_bitwise(fa, inclusions, nsweep):
     v, d, src, rec, eik = inclusions
+    import torch
+
     print(f"160x320x320, 256 sources, nsweep {nsweep}: GPU", flush=True)
     tt, _, vz, sets = eik.solve_tensors(src, nsweep=nsweep)
     assert (sets == nsweep).all()
     got = tt[PICK].cpu().numpy()
-    del tt
+    print("the persistent schedule, every source", flush=True)
+    tp, _, vzp, setsp = eik.solve_tensors(src, nsweep=nsweep, schedule="persistent")
+    assert torch.equal(_bits(tp), _bits(tt))
+    assert np.array_equal(vzp, vz) and np.array_equal(setsp, sets)
+    del tt, tp
     _free()
     print(f"oracle, sources {PICK}", flush=True)
     ref, _, rvz, _ = oracle.solve3d(1.0 / v, *d, src[PICK], nsweep=nsweep, order=1,
