@@ -88,7 +88,7 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
     double t2d1 = kBig, t2d2 = kBig, t2d3 = kBig;
     {
         const double vref = fmin(s000, s001);  // ZX plane
-        if (tv < te + dx * vref && te < tv + dz * vref) {
+        if ((tv < te + dx * vref) & (te < tv + dz * vref)) {
             const double ta = tev + te - tv, tb = tev - te + tv, q = ta - tb;
             t2d1 = div_fixed((tb * dz2i + ta * dx2i) +
                                  sqrt_rn(4.0 * (vref * vref) * A.szx - A.dz2dx2 * (q * q)),
@@ -97,7 +97,7 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
     }
     {
         const double vref = fmin(s000, s010);  // ZY plane
-        if (tv < tn + dy * vref && tn < tv + dz * vref) {
+        if ((tv < tn + dy * vref) & (tn < tv + dz * vref)) {
             const double ta = tv - tn + tnv, tb = tn - tv + tnv, q = ta - tb;
             t2d2 = div_fixed((ta * dz2i + tb * dy2i) +
                                  sqrt_rn(4.0 * (vref * vref) * A.szy - A.dz2dy2 * (q * q)),
@@ -106,7 +106,7 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
     }
     {
         const double vref = fmin(s000, s100);  // XY plane
-        if (te < tn + dy * vref && tn < te + dx * vref) {
+        if ((te < tn + dy * vref) & (tn < te + dx * vref)) {
             const double ta = te - tn + ten, tb = tn - te + ten, q = ta - tb;
             t2d3 = div_fixed((ta * dx2i + tb * dy2i) +
                                  sqrt_rn(4.0 * (vref * vref) * A.sxy - A.dx2dy2 * (q * q)),
@@ -127,7 +127,12 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
         double t3 = A.dz2dx2 * (q1 * q1);
         t3 += A.dz2dy2 * (q2 * q2);
         t3 += A.dx2dy2 * (q3 * q3);
-        if (t2 >= t3) t3d = div_fixed(tb * dz2i + ta * dx2i + tc * dy2i + sqrt_rn(t2 - t3), A.dsum, A.rsum);
+        // every lane of the branch takes the root (of 1.0 where the test fails), so the
+        // test selects the result instead of opening another branch
+        const bool ok3 = t2 >= t3;
+        const double r3 = sqrt_rn(ok3 ? t2 - t3 : 1.0);
+        const double v3 = div_fixed(tb * dz2i + ta * dx2i + tc * dy2i + r3, A.dsum, A.rsum);
+        t3d = ok3 ? v3 : kBig;
     }
 
     const double tnew = fmin(fmin(fmin(t0, t1d), t2d), t3d);

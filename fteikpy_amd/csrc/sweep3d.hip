@@ -383,7 +383,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
             bool changed_now = false, cchanged = false;
             double t_prev = 0.0;  // LOG: time before the change
             const int pslot = cslot == 0 ? kRing - 1 : cslot - 1;
-            if (inb && p >= 0 && p < na) {
+            if (inb && (unsigned)p < (unsigned)na) {
                 double *cur = ring[cslot];
                 const double *prv = ring[pslot];
                 uint8_t *curc = ringc[cslot] + tblk;
@@ -412,10 +412,8 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
                 }
                 const unsigned cs = curc[self];
                 const unsigned cn = (cs & keep) | (changed_now ? pbit : 0u);
-                if (cn != cs) {
-                    curc[self] = (uint8_t)cn;
-                    cchanged = true;
-                }
+                curc[self] = (uint8_t)cn;  // (unchanged bytes rewritten: no branch)
+                cchanged = cn != cs;
             }
             hist = (hist << 1) | (changed_now ? 1u : 0u);
             chist = (chist << 1) | (cchanged ? 1u : 0u);
