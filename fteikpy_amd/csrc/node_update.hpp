@@ -64,8 +64,10 @@ __device__ __forceinline__ double min4(double a, double b, double c, double d) {
 // diagonal ones; sXYZ: slowness of the cell at (a-1 | a') x (b-1 | b') x (c-1 | c'),
 // with ' the downwind-side cell clamped to the grid (the reference's max(i-1,0) /
 // min(i, n-2) pair).  Returns the new value; *which receives the winning candidate.
-// Args derives from Stencil3.
-template <class Args>
+// Args derives from Stencil3.  BF: every lane computes the plane-wave candidates and their
+// tests select them (no branches on the dependent chain of a wave alone on its SIMD);
+// otherwise a branch skips a candidate that no lane of the wave admits.  Same results.
+template <bool BF = false, class Args>
 __device__ __forceinline__ double node_update(const Args &A, double t0, double tv, double te,
                                               double tn, double tev, double ten, double tnv,
                                               double tnve, double s000, double s001, double s010,
@@ -88,29 +90,38 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
     double t2d1 = kBig, t2d2 = kBig, t2d3 = kBig;
     {
         const double vref = fmin(s000, s001);  // ZX plane
-        if ((tv < te + dx * vref) & (te < tv + dz * vref)) {
+        const bool c = (tv < te + dx * vref) & (te < tv + dz * vref);
+        if constexpr (BF) {
             const double ta = tev + te - tv, tb = tev - te + tv, q = ta - tb;
-            t2d1 = div_fixed((tb * dz2i + ta * dx2i) +
-                                 sqrt_rn(4.0 * (vref * vref) * A.szx - A.dz2dx2 * (q * q)),
-                             A.szx, A.rzx);
+            const double v = div_fixed((tb * dz2i + ta * dx2i) + sqrt_rn(c ? 4.0 * (vref * vref) * A.szx - A.dz2dx2 * (q * q) : 1.0), A.szx, A.rzx);
+            t2d1 = c ? v : kBig;
+        } else if (c) {
+            const double ta = tev + te - tv, tb = tev - te + tv, q = ta - tb;
+            t2d1 = div_fixed((tb * dz2i + ta * dx2i) + sqrt_rn(4.0 * (vref * vref) * A.szx - A.dz2dx2 * (q * q)), A.szx, A.rzx);
         }
     }
     {
         const double vref = fmin(s000, s010);  // ZY plane
-        if ((tv < tn + dy * vref) & (tn < tv + dz * vref)) {
+        const bool c = (tv < tn + dy * vref) & (tn < tv + dz * vref);
+        if constexpr (BF) {
             const double ta = tv - tn + tnv, tb = tn - tv + tnv, q = ta - tb;
-            t2d2 = div_fixed((ta * dz2i + tb * dy2i) +
-                                 sqrt_rn(4.0 * (vref * vref) * A.szy - A.dz2dy2 * (q * q)),
-                             A.szy, A.rzy);
+            const double v = div_fixed((ta * dz2i + tb * dy2i) + sqrt_rn(c ? 4.0 * (vref * vref) * A.szy - A.dz2dy2 * (q * q) : 1.0), A.szy, A.rzy);
+            t2d2 = c ? v : kBig;
+        } else if (c) {
+            const double ta = tv - tn + tnv, tb = tn - tv + tnv, q = ta - tb;
+            t2d2 = div_fixed((ta * dz2i + tb * dy2i) + sqrt_rn(4.0 * (vref * vref) * A.szy - A.dz2dy2 * (q * q)), A.szy, A.rzy);
         }
     }
     {
         const double vref = fmin(s000, s100);  // XY plane
-        if ((te < tn + dy * vref) & (tn < te + dx * vref)) {
+        const bool c = (te < tn + dy * vref) & (tn < te + dx * vref);
+        if constexpr (BF) {
             const double ta = te - tn + ten, tb = tn - te + ten, q = ta - tb;
-            t2d3 = div_fixed((ta * dx2i + tb * dy2i) +
-                                 sqrt_rn(4.0 * (vref * vref) * A.sxy - A.dx2dy2 * (q * q)),
-                             A.sxy, A.rxy);
+            const double v = div_fixed((ta * dx2i + tb * dy2i) + sqrt_rn(c ? 4.0 * (vref * vref) * A.sxy - A.dx2dy2 * (q * q) : 1.0), A.sxy, A.rxy);
+            t2d3 = c ? v : kBig;
+        } else if (c) {
+            const double ta = te - tn + ten, tb = tn - te + ten, q = ta - tb;
+            t2d3 = div_fixed((ta * dx2i + tb * dy2i) + sqrt_rn(4.0 * (vref * vref) * A.sxy - A.dx2dy2 * (q * q)), A.sxy, A.rxy);
         }
     }
     const double t2d = fmin(fmin(t2d1, t2d2), t2d3);

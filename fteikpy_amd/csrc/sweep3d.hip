@@ -150,7 +150,7 @@ __device__ __forceinline__ void step_sync() {
 
 // pre < 0: the brick tests each wave's source for skipping itself; else bit w of pre says
 // whether wave w's source is active (the caller tested it, the same test).
-template <bool GRAD, bool LOG, int W>
+template <bool GRAD, bool LOG, int W, bool BF = false>
 __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int g0, int nact,
                       int pre = -1) {
     constexpr int kThreads = 64 * W;
@@ -393,7 +393,7 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
                 const double t0 = cur[tblk + self];
                 if (!may_skip || up != 0u) {
                     int which;
-                    const double tnew = node_update(
+                    const double tnew = node_update<BF>(
                         C, t0, prv[tblk + self], cur[tblk + up_b], cur[tblk + up_c],
                         prv[tblk + up_b], cur[tblk + up_bc], prv[tblk + up_c], prv[tblk + up_bc],
                         sp[0], sp[1], sp[2], sp[3], sc[0], sc[1], sc[2], &which);
@@ -510,7 +510,9 @@ __device__ void brick(const SweepArgs &A, int pass, int BA, int BB, int BC, int 
 // The logged variant (history writes) needs a few VGPRs over the 168 that 3 waves per SIMD
 // allow: held to 168 it spills 4 (12 bytes per lane) and runs 4% faster than at 2 waves
 // (8-source workgroups are held to 2 waves by their LDS anyway).
-template <bool GRAD, bool LOG, int W>
+// BF: the node update without branches around the plane-wave candidates, for solves of so
+// few sources that each wave runs alone on its SIMD (one source per workgroup).
+template <bool GRAD, bool LOG, int W, bool BF = false>
 __global__ __launch_bounds__(64 * W)
 __attribute__((amdgpu_waves_per_eu((LOG && W <= 4) ? 3 : 1, (LOG && W <= 4) ? 3 : 10)))
 void sweep3d_diag_kernel(const SweepArgs A) {
@@ -520,7 +522,7 @@ void sweep3d_diag_kernel(const SweepArgs A) {
     else
         decode_brick(A.diag, blockIdx.x, A.nba, A.nbb, A.nbc, &BA, &BB, &BC);
     // the count of sources still changing is read with the brick's first loads
-    brick<GRAD, LOG, W>(A, pass, BA, BB, BC, blockIdx.y * W, -1);
+    brick<GRAD, LOG, W, BF>(A, pass, BA, BB, BC, blockIdx.y * W, -1);
 }
 
 // Batch of models: launch l of a sweep set covers the bricks of launch l of every
@@ -564,6 +566,9 @@ __global__ void log_mark_kernel(HistHeader *hdr, unsigned long long *seg,
     if (P < hdr->max_pass) seg[P + 1] = *count;
     hdr->npass = P + 1;
 }
+
+// sources up to which the node update runs without branches (see solve3d_impl)
+constexpr int kBranchFreeSources = 4;
 
 // a grid-barrier wait longer than this gives up (the launch then ends and reports it)
 constexpr unsigned long long kBarWaitSeconds = 5;
@@ -1007,7 +1012,13 @@ int set_sweep_groups(SetGraph &s, unsigned gy) {
 }
 
 template <bool G, int W>
-void launch_diag_t(dim3 grid, hipStream_t st, const SweepArgs &a) {
+void launch_diag_t(dim3 grid, hipStream_t st, const SweepArgs &a, bool bf) {
+    if constexpr (W == 1) {
+        if (bf) {
+            hipLaunchKernelGGL((sweep3d_diag_kernel<G, false, 1, true>), grid, dim3(64), 0, st, a);
+            return;
+        }
+    }
     hipLaunchKernelGGL((sweep3d_diag_kernel<G, false, W>), grid, dim3(64 * W), 0, st, a);
 }
 
@@ -1341,6 +1352,11 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
         if (!ov || ov->task.size() > 8 * face_tiles(nz, nx, ny)) return FTEIK_ERR_ARG;
         a.tasks = w.tasks;
     }
+    // a few sources, one per workgroup: each wave runs alone on its SIMD, and the node update
+    // without branches around its plane-wave candidates shortens its dependent chain (one
+    // source 200^3: 27.7 against 28.3 ms); with more waves sharing a SIMD those branches
+    // skip work (256 sources: 3.75 against 3.28 s branch-free)
+    const bool bf = nsrc <= kBranchFreeSources && group == 1;
     // persistent schedule: one cooperative launch per set over nentry launch-list entries
     const int pblocks = schedule == 0 ? pers_blocks(want_grad != 0, group) : 0;
     if (schedule == 0 && (pblocks <= 0 || hv)) return FTEIK_ERR_UNSUPPORTED;
@@ -1431,7 +1447,7 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
             for (int l = 0; l < ov->launches(); ++l) {
                 a.diag = ov->first[l];
                 FTEIK_DISPATCH_W(launch_diag_t, want_grad != 0, group,
-                                 dim3(ov->first[l + 1] - ov->first[l], gy), stream, a);
+                                 dim3(ov->first[l + 1] - ov->first[l], gy), stream, a, bf);
             }
         } else {
             for (int p = 0; p < 8; ++p) {
@@ -1443,7 +1459,7 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
                                          stream, a);
                     } else {
                         FTEIK_DISPATCH_W(launch_diag_t, want_grad != 0, group, dim3(nb[d], gy),
-                                         stream, a);
+                                         stream, a, bf);
                     }
                 }
                 if (hv)
