@@ -130,9 +130,12 @@ __device__ __forceinline__ double node_update(const Args &A, double t0, double t
     double t3d = kBig;
     if (fmin(t1d, t2d) > fmax(fmax(tv, te), tn)) {
         const double vref = s000;
-        const double ta = te - 0.5 * tn + 0.5 * ten - 0.5 * tv + 0.5 * tev - tnv + tnve;
-        const double tb = tv - 0.5 * tn + 0.5 * tnv - 0.5 * te + 0.5 * tev - ten + tnve;
-        const double tc = tn - 0.5 * te + 0.5 * ten - 0.5 * tv + 0.5 * tnv - tev + tnve;
+        // the reference's left-to-right sums te - 0.5 tn + 0.5 ten - ...: each halved term
+        // is exact, so adding it in an FMA rounds the same sum once, as the separate
+        // product and addition do (for times that are 0 or at least 2^-1021)
+        const double ta = fma(0.5, tev, fma(-0.5, tv, fma(0.5, ten, fma(-0.5, tn, te)))) - tnv + tnve;
+        const double tb = fma(0.5, tev, fma(-0.5, te, fma(0.5, tnv, fma(-0.5, tn, tv)))) - ten + tnve;
+        const double tc = fma(0.5, tnv, fma(-0.5, tv, fma(0.5, ten, fma(-0.5, te, tn)))) - tev + tnve;
         const double t2 = vref * vref * A.dsum * 9.0;
         const double q1 = ta - tb, q2 = tb - tc, q3 = ta - tc;
         double t3 = A.dz2dx2 * (q1 * q1);
