@@ -228,7 +228,7 @@ def main():
     # same launches (the library's count for this grid, brick height and the default
     # overlapped schedule), until its last source converged
     tile = args.tile or (tune["tile_a"] if tune else 0)
-    group = args.group or (tune["group"] if tune else (2 if nsrc >= 8 else 1))
+    group = args.group or (tune["group"] if tune else 0)  # 0: the library's default
     sched = tune["schedule"] if tune else "overlapped"
     per_set = fteikpy_amd._lib.lib().fteik3d_sweep_launches(
         nz, nx, ny, nsrc, tile, {"overlapped": 2, "diagonal": 1}[sched])

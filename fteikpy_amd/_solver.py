@@ -5,7 +5,9 @@ for a batch of sources).  Keyword-only extensions:
 
 - ``nsweep=None`` (or <= 0): sweep until a set changes no node, at most ``max_sets`` sets;
 - ``tile_a``: z-extent of the 3D sweep bricks (0 = library default);
-- ``group``: sources per workgroup (0 = default; 1, 2, 4, 8);
+- ``group``: sources per workgroup (0 = default; 1, 2, 4, 8); the default is 1 below 8
+  sources, else 2, except from 32 sources, where the first 2 sweep sets run 4 per
+  workgroup (every source sweeps nearly every brick there) and the later ones 2;
   with both left at 0 (and ``schedule`` at None), a choice recorded by
   ``fteikpy_amd.autotune.tune3d`` for this grid shape and number of sources is used;
 - ``schedule``: "overlapped" (each launch runs the bricks, of any pass, whose inputs are

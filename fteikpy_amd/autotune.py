@@ -117,7 +117,11 @@ def load(path):
 
 def default3d(nsrc):
     """The library's defaults for nsrc sources (fteik3d_solve with tile_a = group = 0):
-    brick height 0 (``default_tile``), 2 sources per workgroup from 8 sources."""
+    brick height 0 (``default_tile``); group 0 from 32 sources (4 sources per workgroup in
+    the first 2 sweep sets, 2 in the later ones), else 2 sources per workgroup from 8
+    sources, 1 below."""
+    if nsrc >= 32:
+        return 0, 0
     return 0, (2 if nsrc >= 8 else 1)
 
 
@@ -152,6 +156,9 @@ def tune3d(eik, sources, *, tiles=TILES, groups=GROUPS, schedules=SCHEDULES, max
     nz, nx, ny = eik.shape
     tiles = sorted({min(int(t), nz + 1) if t > 0 else (nz + 1 if t < 0 else 0) for t in tiles})
     groups = [g for g in groups if g <= max(1, n)] or [1]
+    dt, dg = default3d(n)
+    if dg not in groups:
+        groups = [dg] + groups  # the library's default group is always a candidate
     out = torch.empty((n, nz + 1, nx + 1, ny + 1), dtype=torch.float64,
                       device=torch.device("cuda", torch.cuda.current_device()))
     schedules = [s for s in schedules if s in SCHEDULES] or [SCHEDULES[0]]
@@ -173,8 +180,7 @@ def tune3d(eik, sources, *, tiles=TILES, groups=GROUPS, schedules=SCHEDULES, max
         timings[(tile, group, sched)] = best
         return best
 
-    dt, dg = default3d(n)
-    g0 = dg if dg in groups else groups[0]
+    g0 = dg
     run(tiles[0], g0)  # warm-up: first launches, graph machinery
     timings.clear()
     best_tile = min(tiles, key=lambda t: run(t, g0))

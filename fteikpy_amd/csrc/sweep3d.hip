@@ -570,6 +570,12 @@ __global__ void log_mark_kernel(HistHeader *hdr, unsigned long long *seg,
 // sources up to which the node update runs without branches (see solve3d_impl)
 constexpr int kBranchFreeSources = 4;
 
+// default group (0): from kWideFirstSources sources the first kDenseSets sweep sets run 4
+// sources per workgroup, the later ones 2 (see solve3d_impl)
+constexpr int kWideFirstSources = 32;
+constexpr int kDenseSets = 2;
+constexpr int kWideGroup = 4;
+
 // a grid-barrier wait longer than this gives up (the launch then ends and reports it)
 constexpr unsigned long long kBarWaitSeconds = 5;
 
@@ -1316,11 +1322,14 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
     if ((long long)nx * ny > INT_MAX) return FTEIK_ERR_ARG;  // cell-plane offsets are 32-bit
     if (workspace_bytes < fteik3d_workspace_size(nz, nx, ny, nsrc)) return FTEIK_ERR_WORKSPACE;
     if (nsweep <= 0 && max_sets < 1) return FTEIK_ERR_ARG;
+    if (schedule < 0 || schedule > 2) return FTEIK_ERR_ARG;
     // defaults measured on MI355X (160x320x320, 256 sources): 2 sources per workgroup
-    // beat 4 and 8 (a wider workgroup waits at every step for its slowest source)
+    // beat 4 and 8 (a wider workgroup waits at every step for its slowest source), except
+    // in the first kDenseSets sets, where every source sweeps nearly every brick and 4 per
+    // workgroup win (set 1 489 against 530 ms, set 2 432 against 452)
+    const bool wide_first = group == 0 && nsrc >= kWideFirstSources && schedule != 0 && !hv;
     if (group == 0) group = nsrc >= 8 ? 2 : 1;
     if (group != 1 && group != 2 && group != 4 && group != 8) return FTEIK_ERR_ARG;
-    if (schedule < 0 || schedule > 2) return FTEIK_ERR_ARG;
     hipStream_t stream = (hipStream_t)stream_;
 
     std::vector<SrcParam> prm(nsrc);
@@ -1429,24 +1438,41 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
         }
     }
 
-    // source groups launched per brick: all of them, until a converging solve finds that at
-    // most half are still active (their sources compacted to the front of the list), then
-    // those (the others' workgroups would only read the count and leave)
-    unsigned gy = (unsigned)gmax;
-    auto enqueue_set = [&]() -> int {
+    // The sets run in at most two phases, each with its own source group and captured set:
+    // the first kDenseSets sets at kWideGroup sources per workgroup when wide_first, every
+    // set at `group` otherwise.  Source groups launched per brick (gy): all of them, until
+    // a converging solve finds that at most half are still active (their sources
+    // compacted to the front of the list), then those (the others' workgroups would only
+    // read the count and leave).
+    struct Phase {
+        int group = 0;
+        unsigned gy = 0;
+        SetGraphKey key{};
+        std::unique_ptr<SetGraph> gr;
+    };
+    Phase ph[2];
+    ph[0].group = wide_first ? kWideGroup : group;
+    ph[1].group = group;
+    for (Phase &p : ph) p.gy = (unsigned)ceil_div(nsrc, p.group);
+    auto phase_of = [&](int set) -> Phase & {
+        return wide_first && set < kDenseSets ? ph[0] : ph[1];
+    };
+    auto enqueue_set = [&](const Phase &ps) -> int {
+        const int grp = ps.group;
+        const unsigned gy = ps.gy;
         if (schedule == 0) {
             const long long nt = (long long)sizeof(unsigned) * nentry;
             hipLaunchKernelGGL(zero_bytes_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0,
                                stream, (uint8_t *)w.ticket, nt);
             FTEIK_CHECK(hipGetLastError());
             hipError_t err = hipSuccess;
-            FTEIK_DISPATCH_W(launch_coop_t, want_grad != 0, group, coop_blocks, stream, a,
+            FTEIK_DISPATCH_W(launch_coop_t, want_grad != 0, grp, coop_blocks, stream, a,
                              ov ? w.first : nullptr, nentry, gy, w.ticket, bar, &err);
             FTEIK_CHECK(err);
         } else if (ov) {
             for (int l = 0; l < ov->launches(); ++l) {
                 a.diag = ov->first[l];
-                FTEIK_DISPATCH_W(launch_diag_t, want_grad != 0, group,
+                FTEIK_DISPATCH_W(launch_diag_t, want_grad != 0, grp,
                                  dim3(ov->first[l + 1] - ov->first[l], gy), stream, a, bf);
             }
         } else {
@@ -1455,10 +1481,10 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
                 for (int d = 0; d < a.ndiag; ++d) {
                     a.diag = d;
                     if (hv) {
-                        FTEIK_DISPATCH_W(launch_diag_log_t, false, group, dim3(nb[d], gy),
-                                         stream, a);
+                        FTEIK_DISPATCH_W(launch_diag_log_t, false, grp, dim3(nb[d], gy), stream,
+                                         a);
                     } else {
-                        FTEIK_DISPATCH_W(launch_diag_t, want_grad != 0, group, dim3(nb[d], gy),
+                        FTEIK_DISPATCH_W(launch_diag_t, want_grad != 0, grp, dim3(nb[d], gy),
                                          stream, a, bf);
                     }
                 }
@@ -1472,65 +1498,69 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
         return hip_status(hipGetLastError());
     };
 
-    // the sweep set captured once, instantiated on first use, or taken from the cache of
-    // earlier solves' sets (the legacy default stream cannot be captured; the persistent
-    // schedule and logged solves are not kept)
+    // each phase's sweep set captured once, instantiated on first use, or taken from the
+    // cache of earlier solves' sets (the legacy default stream cannot be captured; the
+    // persistent schedule and logged solves are not kept)
     const bool graphs = use_graph && stream && schedule != 0;  // cooperative launches run direct
     const bool keep = graphs && !hv && schedule != 0;
-    SetGraphKey key{};
-    std::unique_ptr<SetGraph> gr;
+    const int nphase = wide_first ? 2 : 1;
     if (keep) {
         int dev = 0;
         FTEIK_CHECK(hipGetDevice(&dev));
-        key.dev = dev;
-        key.slow = slow;
-        key.tt = tt;
-        key.sgn = want_grad ? sgn : nullptr;
-        key.sets_done = sets_done;
-        key.ws = workspace;
-        key.nz = nz; key.nx = nx; key.ny = ny;
-        key.nsrc = nsrc;
-        key.tile_a = a.tile_a;
-        key.group = group;
-        key.grad = want_grad ? 1 : 0;
-        key.schedule = schedule;
-        key.overlapped = ov ? 1 : 0;
-        std::memcpy(&key.dz, &dz, sizeof dz);
-        std::memcpy(&key.dx, &dx, sizeof dx);
-        std::memcpy(&key.dy, &dy, sizeof dy);
-        gr = graph_checkout(key);
+        for (int i = 2 - nphase; i < 2; ++i) {
+            SetGraphKey &key = ph[i].key;
+            key.dev = dev;
+            key.slow = slow;
+            key.tt = tt;
+            key.sgn = want_grad ? sgn : nullptr;
+            key.sets_done = sets_done;
+            key.ws = workspace;
+            key.nz = nz; key.nx = nx; key.ny = ny;
+            key.nsrc = nsrc;
+            key.tile_a = a.tile_a;
+            key.group = ph[i].group;
+            key.grad = want_grad ? 1 : 0;
+            key.schedule = schedule;
+            key.overlapped = ov ? 1 : 0;
+            std::memcpy(&key.dz, &dz, sizeof dz);
+            std::memcpy(&key.dx, &dx, sizeof dx);
+            std::memcpy(&key.dy, &dy, sizeof dy);
+            ph[i].gr = graph_checkout(key);
+        }
     }
-    auto run_set = [&]() -> int {
+    auto run_set = [&](int set) -> int {
+        Phase &ps = phase_of(set);
         if (graphs) {
+            std::unique_ptr<SetGraph> &gr = ps.gr;
             if (!gr) {
                 gr.reset(new SetGraph);
-                gr->key = key;
+                gr->key = ps.key;
                 FTEIK_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-                int st = enqueue_set();
+                int st = enqueue_set(ps);
                 hipError_t e = hipStreamEndCapture(stream, &gr->g.graph);
                 if (st) return st;
                 FTEIK_CHECK(e);
                 FTEIK_CHECK(hipGraphInstantiate(&gr->g.exec, gr->g.graph, nullptr, nullptr, 0));
-                gr->gy = gy;
+                gr->gy = ps.gy;
                 if (keep)
                     if (int st2 = find_sweep_nodes(*gr)) return st2;
-            } else if (gr->gy != gy) {
+            } else if (gr->gy != ps.gy) {
                 // a kept set still launches the groups its last solve ended with
                 if (gr->done) FTEIK_CHECK(hipEventSynchronize(gr->done));
-                if (int st2 = set_sweep_groups(*gr, gy)) return st2;
+                if (int st2 = set_sweep_groups(*gr, ps.gy)) return st2;
             }
             return hip_status(hipGraphLaunch(gr->g.exec, stream));
         }
-        return enqueue_set();
+        return enqueue_set(ps);
     };
 
     int status = 0;
     if (nsweep > 0) {
-        for (int s = 0; s < nsweep && !status; ++s) status = run_set();
+        for (int s = 0; s < nsweep && !status; ++s) status = run_set(s);
     } else {
         int nact = nsrc;
         for (int s = 0; s < max_sets && nact > 0 && !status; ++s) {
-            status = run_set();
+            status = run_set(s);
             if (status) break;
             unsigned gave_up = 0;
             hipError_t e = hipMemcpyAsync(&nact, w.nlist, sizeof(int), hipMemcpyDeviceToHost, stream);
@@ -1541,15 +1571,17 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
             status = hip_status(e);
             if (!status && gave_up) status = FTEIK_ERR_TIMEOUT;
             if (status) break;
-            const unsigned need = (unsigned)std::max(1, ceil_div(nact, group));
-            if (!status && nact > 0 && 2 * need <= gy) {
-                if (gr && gr->known) {
-                    status = set_sweep_groups(*gr, need);
-                } else if (gr) {
-                    status = set_graph_groups(gr->g.graph, gr->g.exec, gy, need);
-                    gr->gy = need;
+            // the next set's phase launches only the groups that still hold active sources
+            Phase &nx = phase_of(s + 1);
+            const unsigned need = (unsigned)std::max(1, ceil_div(nact, nx.group));
+            if (nact > 0 && 2 * need <= nx.gy) {
+                if (nx.gr && nx.gr->known) {
+                    status = set_sweep_groups(*nx.gr, need);
+                } else if (nx.gr) {
+                    status = set_graph_groups(nx.gr->g.graph, nx.gr->g.exec, nx.gy, need);
+                    nx.gr->gy = need;
                 }
-                gy = need;
+                nx.gy = need;
             }
         }
     }
@@ -1562,11 +1594,14 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
         status = hip_status(e);
         if (!status && gave_up) status = FTEIK_ERR_TIMEOUT;
     }
-    if (!status && keep && gr && gr->known) {
-        // kept for later solves with the same arguments, once this one's replays are queued
-        if (!gr->done) FTEIK_CHECK(hipEventCreateWithFlags(&gr->done, hipEventDisableTiming));
-        FTEIK_CHECK(hipEventRecord(gr->done, stream));
-        graph_checkin(std::move(gr));
+    for (Phase &ps : ph) {
+        if (!status && keep && ps.gr && ps.gr->known) {
+            // kept for later solves with the same arguments, once this one's replays are queued
+            if (!ps.gr->done)
+                FTEIK_CHECK(hipEventCreateWithFlags(&ps.gr->done, hipEventDisableTiming));
+            FTEIK_CHECK(hipEventRecord(ps.gr->done, stream));
+            graph_checkin(std::move(ps.gr));
+        }
     }
     if (!status && hv) {
         unsigned long long count = 0;

@@ -61,7 +61,9 @@ size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc);
  * want_grad record the upwind direction of every update (needed by fteik3d_gradient)
  * tile_a    nodes per brick along z (0 = library default: from 32 sources 20..28 planes,
  *           the fewest steps down a column; 16 from 16 sources; else 8)
- * group     sources per workgroup (0 = default; 1, 2, 4, 8)
+ * group     sources per workgroup (1, 2, 4, 8; 0 = default: 1 below 8 sources, else 2,
+ *           except from 32 sources with schedule 1 or 2, where the first 2 sweep sets run
+ *           4 per workgroup and the later ones 2; the group does not change the result)
  * schedule  2: overlapped passes: each launch runs every brick, of any pass, whose
  *           upwind bricks of its pass and whose tile's readers of the previous pass ran
  *           in earlier launches (508 launches per set instead of 688 for 160x320x320

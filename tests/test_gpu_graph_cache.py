@@ -102,3 +102,32 @@ def test_kept_set_graph_replays_bitwise(device, tile_a, group, schedule, grad):
     held, h2, m2 = _cache()
     assert held == 2 and (h2, m2) == (h, m + 1)
     assert _cache(clear=True)[0] == 0
+
+
+def test_default_group_two_phases(device):
+    """group 0 from 32 sources runs the first 2 sets at 4 sources per workgroup and the
+    later ones at 2, each phase from its own kept graph: the same times, signs and set
+    counts as groups 2 and 4 throughout, at nsweep 1-3 and converged, with and without
+    graphs."""
+    import torch
+
+    shape, d = (22, 19, 27), (7.0, 9.0, 8.0)
+    v = _rough(shape, 43)
+    rng = np.random.default_rng(44)
+    ext = np.array([shape[0] * d[0], shape[1] * d[1], shape[2] * d[2]])
+    src = rng.random((40, 3)) * ext
+    buf = _Buffers(torch, v, src, True)
+    stream = torch.cuda.Stream()
+    _cache(clear=True)
+    for ns in (1, 2, 3, 0):
+        print(f"two phases: nsweep {ns}", flush=True)
+        ref = buf.solve(torch, stream, d, ns, False, 0, 2, 2)
+        for group, graph in ((4, False), (0, False), (0, True), (0, True)):
+            got = buf.solve(torch, stream, d, ns, graph, 0, group, 2)
+            for a, b in zip(got, ref):
+                assert np.array_equal(a, b), (ns, group, graph)
+    held, h, m = _cache()
+    # nsweep 1 captured the first phase only; nsweep 2 reused it and captured no other
+    # (its second set is still in the first phase); nsweep 3 captured the second phase
+    assert held == 2, held
+    assert _cache(clear=True)[0] == 0
