@@ -28,18 +28,19 @@ HBM_PEAK_GBS = 8000.0
 LDS_PEAK_GBS = 128 * 256 * 2.4
 BYTES_PER_NSU = 24  # FP64 tt read + tt write + slowness read per node update
 
-# HBM bytes per launch of the dominant kernel, sweep3d_diag_kernel<false, false, 2>, on the
-# default workload (160 x 320 x 320, 256 sources, the library's default brick height, 27
-# planes here, 2 sources per workgroup, the overlapped schedule): rocprofv3 --pmc
-# FETCH_SIZE and --pmc WRITE_SIZE runs of this script, averaged over the 19812 launches of
-# one converged solve and scaled by a 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x
-# 1.0).  Commands and figures: profiles/sweep3d_traffic__inclusions_160x320x320_256src.md
-PMC_TRAFFIC = {"grid": (160, 320, 320), "sources": 256, "tile_a": 0, "group": 2,
-               "fetch_bytes": 216735.5 * 2.0 * 1024, "write_bytes": 11469.5 * 1.0 * 1024,
+# HBM bytes per launch of the dominant kernel, sweep3d_diag_kernel (<false, false, 4> in the
+# first two sets, <false, false, 2> after them), on the default workload (160 x 320 x 320,
+# 256 sources, the library's default brick height, 27 planes here, and group, the
+# overlapped schedule): rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this
+# script, averaged over the 19812 launches of one converged solve and scaled by a 1 GiB
+# device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Commands and figures:
+# profiles/sweep3d_traffic__inclusions_160x320x320_256src.md
+PMC_TRAFFIC = {"grid": (160, 320, 320), "sources": 256, "tile_a": 0, "group": 0,
+               "fetch_bytes": 208725.2 * 2.0 * 1024, "write_bytes": 11468.6 * 1.0 * 1024,
                # SQ_INSTS_LDS per launch x 64 lanes x 5.4 B, the mean bytes per lane of the
                # step loop's LDS instructions (8 ds_read2_b64, 2 ds_read_b64, 6 ds_write_b64, 18
                # ds_read_u8, 6 ds_write_b8 per two unrolled steps)
-               "lds_bytes": 2.9115596e6 * 64 * 5.4}
+               "lds_bytes": 2.917366e6 * 64 * 5.4}
 
 
 def main():
