@@ -334,6 +334,41 @@ def test_3d_random_models_bitwise(fa):
         assert np.array_equal(tt.cpu().numpy(), ref)
 
 
+def test_3d_many_sources_default_group_bitwise(fa):
+    """From 32 sources the default group runs the first two sweep sets at 4 sources per
+    workgroup and the later ones at 2: 3 random models with 32-44 sources (some on faces
+    and corners), at nsweep 1, 2, 3 and converged, on the overlapped and the pass-by-pass
+    schedule: times and set counts equal the oracle's, gradient directions within the
+    module's bar."""
+    for case in range(3):
+        rng = np.random.default_rng(7300 + case)
+        shape = tuple(int(x) for x in rng.integers(12, 30, 3))
+        p = rng.standard_normal(shape)
+        for ax in range(3):
+            p = (np.roll(p, 1, ax) + p + np.roll(p, -1, ax)) / 3.0
+        v = 2500.0 * np.exp(0.6 * p / np.abs(p).max())
+        d = tuple(rng.uniform(4.0, 12.0, 3))
+        pos = rng.random((int(rng.integers(32, 45)), 3))
+        pos[0] = 0.0
+        pos[1, int(rng.integers(0, 3))] = 1.0
+        srcs = pos * (np.array(shape) * d)
+        eik = fa.Eikonal3D(v, d)
+        for ns in (1, 2, 3, None):
+            print(f"many sources: case {case} nsweep {ns}", flush=True)
+            ref, rgrad, _, k = oracle.solve3d(1.0 / v, *d, srcs, nsweep=ns or 0, grad=True)
+            for sched in ("overlapped", "diagonal"):
+                tt, _, _, sets = eik.solve_tensors(srcs, nsweep=ns, group=0, tile_a=0,
+                                                   schedule=sched)
+                assert np.array_equal(tt.cpu().numpy(), ref), (case, ns, sched)
+                if ns is None:
+                    assert np.array_equal(sets, k), (case, sched)
+            grids = eik.solve(srcs, nsweep=ns, return_gradient=True)
+            for i, g in enumerate(grids):
+                assert np.array_equal(g.grid, ref[i]), (case, ns, i)
+                grad = np.stack([c.grid for c in g.gradient], -1)
+                np.testing.assert_allclose(grad, rgrad[i], rtol=1e-13, atol=1e-15)
+
+
 def test_zero_velocity_block_bitwise(fa):
     """Zero-velocity cells (infinite slowness, no check in the reference either): candidates
     through them come out infinite, or NaN from the fixed-divisor division where the
