@@ -81,7 +81,8 @@ size_t fteik3d_workspace_size(int nz, int nx, int ny, int nsrc);
  * vzero     device [nsrc] slowness of each source cell (output)
  * sets_done device [nsrc] sweep sets executed per source (output)
  * workspace device, at least fteik3d_workspace_size(nsrc) bytes
- * use_graph capture one sweep set as a hipGraph and replay it (1) or launch directly (0);
+ * use_graph 1: capture a sweep set as a hipGraph and replay it (the default group from 32
+ *           sources captures two, one per group); 0: launch directly;
  *           a captured set is kept (see fteik3d_graph_cache) and replayed by later calls
  *           whose launches take the same arguments: same device, slow, tt, sgn, sets_done
  *           and workspace pointers, grid, spacing, source count, brick height, group,
