@@ -369,6 +369,45 @@ def test_3d_many_sources_default_group_bitwise(fa):
                 np.testing.assert_allclose(grad, rgrad[i], rtol=1e-13, atol=1e-15)
 
 
+def test_3d_random_configurations_bitwise(fa):
+    """24 random models of 20-55 cells per side with 1-64 sources, each at a random nsweep,
+    brick height, group, schedule (overlapped, pass by pass, persistent), gradient flag and
+    with or without graph replay: times and set counts equal the oracle's."""
+    import torch
+
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        for case in range(24):
+            rng = np.random.default_rng(91000 + case)
+            shape = tuple(int(x) for x in rng.integers(20, 56, 3))
+            p = rng.standard_normal(shape)
+            for _ in range(int(rng.integers(0, 4))):
+                for ax in range(3):
+                    p = (np.roll(p, 1, ax) + p + np.roll(p, -1, ax)) / 3.0
+            p = p / max(np.abs(p).max(), 1e-12)
+            v = rng.uniform(1000, 5000) * np.exp(rng.uniform(0, 1.2) * p)
+            d = tuple(rng.uniform(1.0, 20.0, 3))
+            pos = rng.random((int(rng.choice([1, 3, 8, 17, 32, 40, 64])), 3))
+            if rng.random() < 0.5:
+                pos[0, int(rng.integers(0, 3))] = float(rng.integers(0, 2))
+            srcs = pos * (np.array(shape) * d)
+            ns = [1, 2, 3, None][int(rng.integers(0, 4))]
+            tile = int(rng.choice([0, 0, 8, 12, 16, 27]))
+            group = int(rng.choice([0, 0, 1, 2, 4]))
+            sched = str(rng.choice(["overlapped", "overlapped", "diagonal", "persistent"]))
+            grad = bool(rng.random() < 0.3)
+            print(f"random configuration {case}: {shape} {len(srcs)} sources nsweep {ns} "
+                  f"tile {tile} group {group} {sched} grad {grad}", flush=True)
+            ref, _, _, k = oracle.solve3d(1.0 / v, *d, srcs, nsweep=ns or 0)
+            eik = fa.Eikonal3D(v, d)
+            tt, _, _, sets = eik.solve_tensors(srcs, nsweep=ns, tile_a=tile, group=group,
+                                               schedule=sched, return_gradient=grad,
+                                               use_graph=bool(case % 2))
+            assert np.array_equal(tt.cpu().numpy(), ref), case
+            if ns is None:
+                assert np.array_equal(sets, k), case
+
+
 def test_zero_velocity_block_bitwise(fa):
     """Zero-velocity cells (infinite slowness, no check in the reference either): candidates
     through them come out infinite, or NaN from the fixed-divisor division where the
