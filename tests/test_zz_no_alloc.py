@@ -54,13 +54,17 @@ CASES = [  # (shape, spacing, sources, nsweep, gradient signs)
     ((22, 19, 27), (7.0, 9.0, 8.0),
      [[3.3, 17.7, 40.1], [140.0, 60.0, 100.0], [147.0, 171.0, 216.0], [0.0, 0.0, 0.0]], 0, True),
     ((9, 12, 10), (10.0, 10.0, 10.0), [[45.0, 61.0, 33.3]], 3, False),
+    # 40 sources: the default group runs its first two sets at 4 sources per workgroup
+    ((14, 17, 12), (6.0, 5.0, 7.0),
+     (np.random.default_rng(8).random((40, 3)) * np.array([84.0, 85.0, 84.0])).tolist(), 0,
+     False),
 ]
 
 
 @pytest.mark.parametrize("order", [0, 1])
 def test_noalloc_matches_workspace_solve_twice(device, order):
-    """Two calls in a row, the second on the smaller problem (order 0) or on the larger one
-    after the smaller (order 1)."""
+    """Calls in a row, the second on the smaller problem (order 0) or on the larger one
+    after the smaller (order 1), with a 40-source problem last or first."""
     from fteikpy_amd import _lib
 
     L = _lib.lib()
