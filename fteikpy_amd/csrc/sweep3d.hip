@@ -1327,7 +1327,7 @@ int solve3d_impl(const double *slow, int nz, int nx, int ny, double dz, double d
     // beat 4 and 8 (a wider workgroup waits at every step for its slowest source), except
     // in the first kDenseSets sets, where every source sweeps nearly every brick and 4 per
     // workgroup win (set 1 489 against 530 ms, set 2 432 against 452)
-    const bool wide_first = group == 0 && nsrc >= kWideFirstSources && schedule != 0 && !hv;
+    const bool wide_first = group == 0 && nsrc >= kWideFirstSources && schedule != 0;
     if (group == 0) group = nsrc >= 8 ? 2 : 1;
     if (group != 1 && group != 2 && group != 4 && group != 8) return FTEIK_ERR_ARG;
     hipStream_t stream = (hipStream_t)stream_;
