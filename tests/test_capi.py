@@ -290,6 +290,22 @@ def test_autotune_records_and_persists(tmp_path):
     autotune.clear()
 
 
+def test_autotune_library_defaults():
+    """default3d states the library's defaults: group 0 (two phases: 4 sources per
+    workgroup in the first two sets, then 2) from 32 sources, 2 from 8, 1 below; the
+    brick height of default_tile."""
+    from fteikpy_amd import autotune
+
+    assert autotune.default3d(256) == (0, 0)
+    assert autotune.default3d(32) == (0, 0)
+    assert autotune.default3d(31) == (0, 2)
+    assert autotune.default3d(8) == (0, 2)
+    assert autotune.default3d(7) == (0, 1)
+    assert autotune.default3d(1) == (0, 1)
+    assert autotune.default_tile(256, 161) == 27 and autotune.default_tile(128, 129) == 26
+    assert autotune.default_tile(16, 161) == 16 and autotune.default_tile(1, 201) == 8
+
+
 def test_cell_plane_limit():
     """In-plane cell offsets of the 3D sweep are 32-bit: a cell plane of more than 2^31 - 1
     cells is refused from the host, before any device work (fake pointers)."""
